@@ -1,0 +1,287 @@
+/*
+ * qphix_amd.h -- C ABI of the gfx950 (MI355X) implementation of the QPhiX
+ * even-odd Dslash, fused BLAS and Krylov solvers.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (QPHIX_OK = 0) and is stream-ordered on the library's HIP stream
+ * (qphix_set_stream). Entry points that return host scalars (reductions,
+ * solvers) synchronise that stream. Field pointers are device pointers returned
+ * by the qphix_alloc_* calls of the geometry they belong to; a geometry fixes
+ * the lattice, the precision (the FT template argument of the reference) and
+ * the gauge compression (compress12).
+ *
+ * Canonical host layouts accepted by the packers are those of the QDP++
+ * packers (include/qphix/qdp_packer_parscalar.h):
+ *   site index  i = cb*Vh + ((t*Nz + z)*Ny + y)*Nxh + x/2, cb = (x+y+z+t)&1
+ *   spinor (one cb)  double[Vh][4 spin][3 colour][2]
+ *   gauge  (full)    double[V][4 mu][3 row][3 col][2]     U_mu(x), no boundary phase
+ *   clover (one cb)  double[Vh][72]  Chroma PrimitiveClovTriang: diag[2][6], offd[2][15][2]
+ *   full clover      double[Vh][2 block][6][6][2]
+ * The reference source lines cited below are in JeffersonLab/qphix.
+ */
+#ifndef QPHIX_AMD_H
+#define QPHIX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference aborts / throws, dslash_body.h:64-95,
+ *      geometry.h:151-154, minvcg.h:92-97, invbicgstab.h:163-230) ---------- */
+enum {
+  QPHIX_OK = 0,
+  QPHIX_EDOMAIN = 1,     /* lattice / argument outside the supported domain   */
+  QPHIX_ENOMEM = 2,      /* device allocation failed                           */
+  QPHIX_EHIP = 3,        /* HIP runtime error                                  */
+  QPHIX_EBREAKDOWN = 4,  /* solver breakdown (rho = 0, <r0,v> = 0, |t| = 0)     */
+  QPHIX_ENOCONV = 5,     /* solver reached MaxIters                            */
+  QPHIX_EINVAL = 6,      /* null / mismatched pointer or handle                */
+  QPHIX_ENODEV = 7       /* no HIP device                                      */
+};
+
+/* precision of a geometry (Geometry<FT,...> FT = double / float / half) */
+enum { QPHIX_F64 = 0, QPHIX_F32 = 1, QPHIX_F16 = 2 };
+
+typedef struct qphix_geom qphix_geom;
+typedef struct qphix_dslash qphix_dslash;
+typedef struct qphix_op qphix_op;
+typedef struct qphix_solver qphix_solver;
+
+const char *qphix_strerror(int status);
+/* message of the last failing call on this host thread */
+const char *qphix_last_error(void);
+
+/* ---- device and stream --------------------------------------------------- */
+int qphix_init(int device);
+/* hip_stream is a hipStream_t; NULL selects the library's own stream */
+int qphix_set_stream(void *hip_stream);
+int qphix_sync(void);
+/* HIP-event timer on the library stream */
+int qphix_timer_start(void);
+int qphix_timer_stop(double *ms);
+
+/* ---- geometry: replaces Geometry<FT,V,S,c12>(latt, By, Bz, NCores, Sy, Sz,
+ *      PadXY, PadXYZ, MinCt) (geometry.h:130-235). Blocking / threading
+ *      arguments have no meaning on the GPU; all four extents must be even. */
+int qphix_geom_create(const int latt[4], int prec, int compress12, qphix_geom **g);
+int qphix_geom_destroy(qphix_geom *g);
+/* bytes[0..3] = spinor, gauge (one cb), clover, full clover allocation sizes */
+int qphix_geom_info(const qphix_geom *g, int latt[4], int *prec, int *compress12,
+                    long *vh, size_t bytes[4]);
+
+/* ---- allocation: allocCBFourSpinor / allocCBGauge / allocCBClov /
+ *      allocCBFullClov / free (geometry.h:270-398). Memory is zeroed. ------- */
+int qphix_alloc_spinor(const qphix_geom *g, void **p);
+int qphix_alloc_gauge(const qphix_geom *g, void **p);
+int qphix_alloc_clover(const qphix_geom *g, void **p);
+int qphix_alloc_full_clover(const qphix_geom *g, void **p);
+int qphix_free(void *p);
+
+/* ---- packers: qdp_pack_spinor / qdp_unpack_spinor / qdp_pack_gauge /
+ *      qdp_pack_clover / qdp_pack_full_clover
+ *      (qdp_packer_parscalar.h:32-146, 148-227, 230-327, 329-461) ---------- */
+int qphix_pack_spinor(const qphix_geom *g, void *dst, const double *src, int cb);
+int qphix_unpack_spinor(const qphix_geom *g, double *dst, const void *src, int cb);
+/* fills both checkerboard gauge arrays (8 links per site incl. back links) */
+int qphix_pack_gauge(const qphix_geom *g, void *u_cb0, void *u_cb1, const double *src);
+int qphix_pack_clover(const qphix_geom *g, void *dst, const double *src, int cb);
+int qphix_unpack_clover(const qphix_geom *g, double *dst, const void *src, int cb);
+int qphix_pack_full_clover(const qphix_geom *g, void *dst, const double *src, int cb);
+
+/* ---- precision conversion: convert() (blas_new_c.h:852-1031) for spinors
+ *      with a scale factor; gauge / clover copies between geometries of the
+ *      same lattice (what the mixed-precision callers build by re-packing). */
+int qphix_convert_spinor(const qphix_geom *g_out, void *out, double scale,
+                         const qphix_geom *g_in, const void *in);
+int qphix_convert_gauge(const qphix_geom *g_out, void *out, const qphix_geom *g_in,
+                        const void *in);
+int qphix_convert_clover(const qphix_geom *g_out, void *out, const qphix_geom *g_in,
+                         const void *in, int full);
+
+/* ---- seeded field preparation on the device (replaces the QDP++ RNG set-up
+ *      and tests/clover_term_qdp_w.h:114-195, 325-442, tests/mesfield.cc:24-64).
+ *      gauge: U_mu(x) = reunit(exp(i*eps*A)), A traceless Hermitian with
+ *      unit-variance Gaussian G: A = (G + G^+)/2 - tr(G + G^+)/6.  */
+int qphix_gen_gauge(const qphix_geom *g, void *u_cb0, void *u_cb1, uint64_t seed, double eps);
+/* Re, Im ~ N(0,1) per site, spin, colour */
+int qphix_gen_spinor(const qphix_geom *g, void *dst, uint64_t seed, int cb);
+enum {
+  QPHIX_CLOV_A = 0,       /* A = diag_mass + clover term (packed Hermitian)  */
+  QPHIX_CLOV_AINV = 1,    /* A^-1 (packed Hermitian)                          */
+  QPHIX_CLOV_TMINV = 2,   /* full blocks (A + i*mu*gamma_5)^-1                */
+  QPHIX_CLOV_ND_UP = 3,   /* full blocks (A^2+mu^2-eps^2)^-1 (A - i mu g5)    */
+  QPHIX_CLOV_ND_DN = 4,   /* full blocks (A^2+mu^2-eps^2)^-1 (A + i mu g5)    */
+  QPHIX_CLOV_ND_OFF = 5   /* eps (A^2+mu^2-eps^2)^-1 (packed Hermitian)       */
+};
+/* Clover-leaf field strength and the Chroma clover term on checkerboard cb,
+ * computed in FP64 from the packed gauge arrays. csw_r / csw_t are the clover
+ * coefficients (isotropic: both c_sw), diag_mass = 4 + m. */
+int qphix_clover_build(const qphix_geom *g, const void *u_cb0, const void *u_cb1,
+                       double csw_r, double csw_t, double diag_mass, int cb, int what,
+                       double mu, double eps, void *out);
+
+/* ---- Dslash objects: Dslash / ClovDslash / TMDslash / TMClovDslash
+ *      constructors (dslash_def.h:27-32, clover_dslash_def.h:25-31,
+ *      tm_dslash_def.h:22-29, tm_clov_dslash_def.h:21-30) --------------- */
+typedef struct {
+  double t_boundary;      /* multiplies the hops across the t boundary       */
+  double aniso_s;         /* spatial hop coefficient                          */
+  double aniso_t;         /* temporal hop coefficient                         */
+  int use_tbc[4];         /* per dimension: apply tbc_phases to every hop     */
+  double tbc_phases[4][2];
+  double mass;            /* TMDslash: Wilson mass (alpha = 4 + mass)         */
+  double twisted_mass;    /* TMDslash: mu                                     */
+  double prec_mass_rho;   /* ClovDslash / TMClovDslash: rho                   */
+} qphix_dslash_params;
+int qphix_dslash_create(const qphix_geom *g, const qphix_dslash_params *p, qphix_dslash **d);
+int qphix_dslash_destroy(qphix_dslash *d);
+
+/* Dslash::dslash, ::dslashAChiMinusBDPsi (dslash_def.h:38-51) */
+int qphix_wilson_dslash(qphix_dslash *d, void *res, const void *psi, const void *u,
+                        int isign, int cb);
+int qphix_wilson_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi,
+                            const void *u, double alpha, double beta, int isign, int cb);
+/* ClovDslash::dslash, ::dslashAChiMinusBDPsi, ::clovMult (clover_dslash_def.h:41-78) */
+int qphix_clov_dslash(qphix_dslash *d, void *res, const void *psi, const void *u,
+                      const void *invclov, int isign, int cb);
+int qphix_clov_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi,
+                          const void *u, const void *clov, double beta, int isign, int cb);
+int qphix_clov_mult(qphix_dslash *d, void *res, const void *chi, const void *clov);
+/* TMDslash::dslash, ::dslashAChiMinusBDPsi, ::two_flav_AChiMinusBDPsi
+ * (tm_dslash_def.h:35-58) */
+int qphix_tm_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign,
+                    int cb);
+int qphix_tm_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi,
+                        const void *u, double alpha, double beta, int isign, int cb,
+                        double musign);
+int qphix_tm_two_flav_achimbdpsi(qphix_dslash *d, void *const res[2],
+                                 const void *const psi[2], const void *const chi[2],
+                                 const void *u, double alpha, double beta, double epsilon,
+                                 int isign, int cb);
+/* TMClovDslash::dslash, ::dslashAChiMinusBDPsi, ::two_flav_AChiMinusBDPsi,
+ * ::two_flav_inverse_clover_term (tm_clov_dslash_def.h:44-109) */
+int qphix_tmclov_dslash(qphix_dslash *d, void *res, const void *psi, const void *u,
+                        const void *const invclov[2], int isign, int cb, int fl);
+int qphix_tmclov_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi,
+                            const void *u, const void *clov, double beta, int isign, int cb,
+                            int fl);
+int qphix_tmclov_two_flav_achimbdpsi(qphix_dslash *d, void *const res[2],
+                                     const void *const psi[2], const void *const chi[2],
+                                     const void *u, const void *clov, double beta,
+                                     double epsilon, int isign, int cb);
+int qphix_tmclov_two_flav_inverse_clover_term(qphix_dslash *d, void *const res[2],
+                                              const void *const psi[2],
+                                              const void *const fcl[2],
+                                              const void *cl_offdiag, int isign);
+
+/* ---- even-odd operators: EvenOddLinearOperator::operator() (linearOp.h:20-35)
+ *      for EvenOddWilsonOperator (wilson.h:49-65), EvenOddCloverOperator
+ *      (clover.h:162-172), EvenOddTMWilsonOperator (twisted_mass.h:62-77),
+ *      EvenOddTMCloverOperator (twisted_mass_clover.h:92-102),
+ *      EvenOddNDTMWilsonReuseOperator (ndtm_reuse_operator.h:65-108),
+ *      EvenOddNDTMCloverReuseOperator (ndtm_reuse_operator_clover.h:115-128) */
+enum {
+  QPHIX_OP_WILSON = 0,
+  QPHIX_OP_CLOVER = 1,
+  QPHIX_OP_TMWILSON = 2,
+  QPHIX_OP_TMCLOVER = 3,
+  QPHIX_OP_NDTMWILSON = 4,
+  QPHIX_OP_NDTMCLOVER = 5
+};
+typedef struct {
+  int kind;
+  double mass;            /* Wilson kinds: alpha = 4 + mass                      */
+  double twisted_mass;    /* TM kinds: mu                                        */
+  double epsilon;         /* ND kinds: flavour splitting                         */
+  double prec_mass_rho;   /* clover kinds: +i rho gamma_5 on the target block;   */
+                          /* TM clover kinds pass the twisted mass here          */
+  qphix_dslash_params hop;
+  const void *u[2];              /* gauge per checkerboard                        */
+  const void *clov[2];           /* A per cb (clover kinds; target cb used)       */
+  const void *invclov[2];        /* A^-1 per cb (clover; other cb used)           */
+  const void *fullclov[2][2];    /* TM clover: [cb][0: +mu, 1: -mu] inverse blocks */
+                                 /* ND clover: [cb][0: up, 1: down] inverse blocks */
+  const void *clov_odiag[2];     /* ND clover: flavour off-diagonal block per cb   */
+} qphix_op_params;
+int qphix_op_create(const qphix_geom *g, const qphix_op_params *p, qphix_op **op);
+int qphix_op_destroy(qphix_op *op);
+int qphix_op_nflav(const qphix_op *op);
+/* res[f], in[f] for f < nflav */
+int qphix_op_apply(qphix_op *op, void *const res[], const void *const in[], int isign,
+                   int target_cb);
+
+/* ---- solvers: AbstractSolver::operator() (abs_solver.h:85-97) for InvCG
+ *      (invcg.h:25-694), InvBiCGStab (invbicgstab.h:18-619),
+ *      InvRichardsonMultiPrec (inv_richardson_multiprec.h:11-320) and
+ *      AbstractMultiSolver (abs_solver.h:218-229) for MInvCG (minvcg.h:17-452) */
+enum { QPHIX_RESID_ABSOLUTE = 0, QPHIX_RESID_RELATIVE = 1 };
+typedef struct {
+  int n_iters;
+  double rsd_sq_final;      /* as the reference reports it (abs for CG/BiCGStab) */
+  unsigned long site_flops;
+  unsigned long mv_apps;
+  int status;               /* QPHIX_OK, QPHIX_EBREAKDOWN or QPHIX_ENOCONV       */
+} qphix_solve_stats;
+int qphix_cg_create(qphix_op *M, int max_iters, qphix_solver **s);
+int qphix_bicgstab_create(qphix_op *M, int max_iters, qphix_solver **s);
+/* outer operator M_outer and inner solver (built on an operator of another
+ * geometry / precision); mdagm = 1 solves M^+M x = rhs */
+int qphix_richardson_create(qphix_op *M_outer, qphix_solver *inner, double delta,
+                            int max_iters, int mdagm, qphix_solver **s);
+int qphix_minvcg_create(qphix_op *M, int max_iters, int max_shifts, qphix_solver **s);
+int qphix_solver_destroy(qphix_solver *s);
+int qphix_solve(qphix_solver *s, void *const x[], const void *const rhs[], double rsd_target,
+                int isign, int verbose, int target_cb, int resid_type,
+                qphix_solve_stats *st);
+/* x[shift * nflav + f]; rsd_sq_final[n_shift] */
+int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[],
+                       int n_shift, const double *shifts, const double *rsd_target,
+                       int isign, int verbose, int target_cb, int *n_iters,
+                       double *rsd_sq_final, unsigned long *site_flops,
+                       unsigned long *mv_apps);
+
+/* ---- fused BLAS: blas_new_c.h functions with num_flav = 1 (line numbers).
+ *      Scalars are doubles, applied in the geometry's arithmetic type;
+ *      reductions accumulate in FP64 and return on the host. ------------- */
+int qphix_copy_spinor(const qphix_geom *g, void *res, const void *src);                 /* 18 */
+int qphix_zero_spinor(const qphix_geom *g, void *res);                                  /* 49 */
+int qphix_axy(const qphix_geom *g, double alpha, const void *x, void *y);              /* 70: y = a x */
+int qphix_ax(const qphix_geom *g, double alpha, void *x);                              /* 102 */
+int qphix_ypeqx(const qphix_geom *g, const void *x, void *y);                          /* 114: y += x */
+int qphix_ymeqx(const qphix_geom *g, const void *x, void *y);                          /* 126: y -= x */
+int qphix_aypx(const qphix_geom *g, double alpha, const void *x, void *y);             /* 139: y = a y + x */
+int qphix_twisted_mass(const qphix_geom *g, const double apimu[2], const void *x, void *y); /* 157 */
+int qphix_two_flav_twisted_mass(const qphix_geom *g, const double apimu[2], double epsilon,
+                                const void *const x[2], void *const y[2]);               /* 176 */
+int qphix_axpy(const qphix_geom *g, double alpha, const void *x, void *y);             /* 214 */
+int qphix_axpby(const qphix_geom *g, double alpha, const void *x, double beta, void *y); /* 247 */
+int qphix_norm2_spinor(const qphix_geom *g, const void *x, double *n2);                 /* 278 */
+int qphix_axpy_norm2(const qphix_geom *g, double alpha, const void *x, void *y, double *n2); /* 312 */
+int qphix_xmy_norm2(const qphix_geom *g, void *res, const void *x, const void *y, double *n2); /* 345: res = x - y */
+int qphix_xmy2_norm2(const qphix_geom *g, const void *x, void *y, double *n2);        /* 360: y = y - x */
+int qphix_rmammp_norm2_rxpap(const qphix_geom *g, void *r, double a, const void *mmp,
+                             void *x, const void *p, double *cp);                        /* 397 */
+int qphix_richardson_rxupdate_norm_r(const qphix_geom *g, void *x, void *r, const void *dx,
+                                     const void *dr, double *n2);                        /* 435 */
+int qphix_bicgstab_xmy(const qphix_geom *g, const void *x, void *y);                   /* 489: y = x - y */
+int qphix_inner_product(const qphix_geom *g, const void *l, const void *r, double out[2]); /* 520 */
+int qphix_inner_product_norm(const qphix_geom *g, const void *l, const void *r,
+                             double out[3]);                                             /* 573 */
+int qphix_bicgstab_p_update(const qphix_geom *g, const void *r, void *p, const void *v,
+                            const double beta[2], const double omega[2]);               /* 627 */
+int qphix_bicgstab_s_update(const qphix_geom *g, const double alpha[2], void *s,
+                            const void *v);                                              /* 679 */
+int qphix_caxpy(const qphix_geom *g, const double alpha[2], const void *x, void *y);   /* 696 */
+int qphix_bicgstab_rxupdate(const qphix_geom *g, void *x, void *r, const void *t,
+                            const void *p, const double omega[2], const double alpha[2],
+                            double *r_norm);                                             /* 736 */
+int qphix_mr_rxupdate(const qphix_geom *g, void *x, void *r, const void *mr,
+                      const double a[2]);                                                /* 796 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QPHIX_AMD_H */

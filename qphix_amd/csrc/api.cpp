@@ -1,0 +1,1247 @@
+// C ABI of the gfx950 QPhiX hot path: geometry, fields, Dslash objects,
+// even-odd operators, Krylov solvers and the fused BLAS. All work is ordered on
+// one HIP stream; calls that return host scalars synchronise it.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qphix_amd.h"
+#include "fields.hpp"
+#include "hop_impl.hpp"
+
+using namespace qphix_amd;
+
+namespace qphix_amd {
+template <int P>
+hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int block, hipStream_t st);
+extern template hipError_t launch_hop_prec<P64>(int, bool, int, const HopArgs &, int, hipStream_t);
+extern template hipError_t launch_hop_prec<P32>(int, bool, int, const HopArgs &, int, hipStream_t);
+extern template hipError_t launch_hop_prec<P16>(int, bool, int, const HopArgs &, int, hipStream_t);
+}  // namespace qphix_amd
+
+struct qphix_geom {
+  GeomDesc d;
+  size_t bytes[4];
+};
+struct qphix_dslash {
+  const qphix_geom *g;
+  qphix_dslash_params p;
+};
+struct qphix_op {
+  const qphix_geom *g;
+  qphix_op_params p;
+  qphix_dslash d;
+  int nflav;
+  void *tmp[2];
+  void *tmp2[2];
+};
+enum { S_CG = 0, S_BICGSTAB = 1, S_RICHARDSON = 2, S_MINVCG = 3 };
+struct qphix_solver {
+  int kind;
+  qphix_op *M;
+  int max_iters;
+  std::vector<void *> work;  // solver-owned spinors
+  // Richardson
+  qphix_solver *inner;
+  double delta;
+  int mdagm;
+  // MInvCG
+  int max_shifts;
+};
+
+// ---------------------------------------------------------------------------
+// state and errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static bool g_init = false;
+static int g_device = 0;
+static hipStream_t g_own_stream = nullptr;
+static hipStream_t g_stream = nullptr;
+static double *g_partials = nullptr;  // device, kMaxBlasBlocks * 3
+static double *g_dres = nullptr;      // device, 4
+static double *g_hres = nullptr;      // pinned host, 4
+static hipEvent_t g_ev0, g_ev1;
+static int g_hop_block = 256;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIPCHK(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return fail(QPHIX_EHIP, "%s: %s", #call, hipGetErrorString(e_));        \
+  } while (0)
+#define ENSURE_INIT()                 \
+  do {                                \
+    if (!g_init) {                    \
+      int r_ = qphix_init(g_device);  \
+      if (r_) return r_;              \
+    }                                 \
+  } while (0)
+#define CHKPTR(p) \
+  do {            \
+    if (!(p)) return fail(QPHIX_EINVAL, "%s: null argument '%s'", __func__, #p); \
+  } while (0)
+#define TRY(x)            \
+  do {                    \
+    int r_ = (x);         \
+    if (r_) return r_;    \
+  } while (0)
+
+extern "C" const char *qphix_strerror(int s) {
+  switch (s) {
+    case QPHIX_OK: return "ok";
+    case QPHIX_EDOMAIN: return "argument outside the supported domain";
+    case QPHIX_ENOMEM: return "device allocation failed";
+    case QPHIX_EHIP: return "HIP runtime error";
+    case QPHIX_EBREAKDOWN: return "solver breakdown";
+    case QPHIX_ENOCONV: return "solver did not converge";
+    case QPHIX_EINVAL: return "invalid argument";
+    case QPHIX_ENODEV: return "no HIP device";
+  }
+  return "unknown status";
+}
+extern "C" const char *qphix_last_error(void) { return g_err.c_str(); }
+
+extern "C" int qphix_init(int device) {
+  if (g_init) return QPHIX_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(QPHIX_ENODEV, "no HIP device visible");
+  if (device < 0 || device >= n) return fail(QPHIX_EDOMAIN, "device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  g_device = device;
+  HIPCHK(hipStreamCreate(&g_own_stream));
+  g_stream = g_own_stream;
+  HIPCHK(hipMalloc(&g_partials, sizeof(double) * kMaxBlasBlocks * 3));
+  HIPCHK(hipMalloc(&g_dres, sizeof(double) * 4));
+  HIPCHK(hipHostMalloc(&g_hres, sizeof(double) * 4, hipHostMallocDefault));
+  HIPCHK(hipEventCreate(&g_ev0));
+  HIPCHK(hipEventCreate(&g_ev1));
+  g_init = true;
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_set_stream(void *s) {
+  ENSURE_INIT();
+  g_stream = s ? (hipStream_t)s : g_own_stream;
+  return QPHIX_OK;
+}
+extern "C" int qphix_sync(void) {
+  ENSURE_INIT();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  HIPCHK(hipGetLastError());
+  return QPHIX_OK;
+}
+extern "C" int qphix_timer_start(void) {
+  ENSURE_INIT();
+  HIPCHK(hipEventRecord(g_ev0, g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_timer_stop(double *ms) {
+  ENSURE_INIT();
+  CHKPTR(ms);
+  HIPCHK(hipEventRecord(g_ev1, g_stream));
+  HIPCHK(hipEventSynchronize(g_ev1));
+  float f = 0;
+  HIPCHK(hipEventElapsedTime(&f, g_ev0, g_ev1));
+  *ms = f;
+  return QPHIX_OK;
+}
+// hop kernel work-group size (multiple of 64, at most 256)
+extern "C" int qphix_set_hop_block(int b) {
+  if (b <= 0 || b > 256 || (b & 63)) return fail(QPHIX_EDOMAIN, "hop block %d must be 64, 128, 192 or 256", b);
+  g_hop_block = b;
+  return QPHIX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// geometry and allocation
+// ---------------------------------------------------------------------------
+extern "C" int qphix_geom_create(const int latt[4], int prec, int c12, qphix_geom **out) {
+  CHKPTR(latt);
+  CHKPTR(out);
+  for (int k = 0; k < 4; k++)
+    if (latt[k] < 2 || (latt[k] & 1))
+      return fail(QPHIX_EDOMAIN, "lattice extent %d in dimension %d must be even and >= 2", latt[k], k);
+  if (prec != QPHIX_F64 && prec != QPHIX_F32 && prec != QPHIX_F16)
+    return fail(QPHIX_EDOMAIN, "unknown precision %d", prec);
+  if (prec == QPHIX_F16 && !c12) return fail(QPHIX_EDOMAIN, "F16 geometries require two-row gauge compression");
+  qphix_geom *g = new qphix_geom;
+  for (int k = 0; k < 4; k++) g->d.L[k] = latt[k];
+  g->d.prec = prec;
+  g->d.c12 = c12 ? 1 : 0;
+  g->d.vh = (long)latt[0] * latt[1] * latt[2] * latt[3] / 2;
+  g->d.vs = (size_t)((g->d.vh + 255) / 256) * 256;
+  const int nr[4] = {kSpinorReals, gauge_reals(c12 != 0), kCloverReals, kFullCloverReals};
+  for (int k = 0; k < 4; k++) {
+    switch (prec) {
+      case QPHIX_F64: g->bytes[k] = field_bytes<P64>(nr[k], g->d.vs); break;
+      case QPHIX_F32: g->bytes[k] = field_bytes<P32>(nr[k], g->d.vs); break;
+      default: g->bytes[k] = field_bytes<P16>(nr[k], g->d.vs); break;
+    }
+  }
+  *out = g;
+  return QPHIX_OK;
+}
+extern "C" int qphix_geom_destroy(qphix_geom *g) {
+  delete g;
+  return QPHIX_OK;
+}
+extern "C" int qphix_geom_info(const qphix_geom *g, int latt[4], int *prec, int *c12, long *vh, size_t bytes[4]) {
+  CHKPTR(g);
+  if (latt)
+    for (int k = 0; k < 4; k++) latt[k] = g->d.L[k];
+  if (prec) *prec = g->d.prec;
+  if (c12) *c12 = g->d.c12;
+  if (vh) *vh = g->d.vh;
+  if (bytes)
+    for (int k = 0; k < 4; k++) bytes[k] = g->bytes[k];
+  return QPHIX_OK;
+}
+
+static int alloc_zeroed(size_t n, void **p) {
+  ENSURE_INIT();
+  CHKPTR(p);
+  void *q = nullptr;
+  if (hipMalloc(&q, n) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(QPHIX_ENOMEM, "hipMalloc of %zu bytes failed", n);
+  }
+  HIPCHK(hipMemsetAsync(q, 0, n, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  *p = q;
+  return QPHIX_OK;
+}
+extern "C" int qphix_alloc_spinor(const qphix_geom *g, void **p) { CHKPTR(g); return alloc_zeroed(g->bytes[0], p); }
+extern "C" int qphix_alloc_gauge(const qphix_geom *g, void **p) { CHKPTR(g); return alloc_zeroed(g->bytes[1], p); }
+extern "C" int qphix_alloc_clover(const qphix_geom *g, void **p) { CHKPTR(g); return alloc_zeroed(g->bytes[2], p); }
+extern "C" int qphix_alloc_full_clover(const qphix_geom *g, void **p) { CHKPTR(g); return alloc_zeroed(g->bytes[3], p); }
+extern "C" int qphix_free(void *p) {
+  if (!p) return QPHIX_OK;
+  ENSURE_INIT();
+  HIPCHK(hipStreamSynchronize(g_stream));
+  HIPCHK(hipFree(p));
+  return QPHIX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// packers, generation, conversion, clover build
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+static int stage_in(DevBuf &b, const double *src, size_t n) {
+  if (hipMalloc(&b.p, n * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(QPHIX_ENOMEM, "staging allocation of %zu doubles failed", n);
+  }
+  HIPCHK(hipMemcpyAsync(b.p, src, n * sizeof(double), hipMemcpyHostToDevice, g_stream));
+  return QPHIX_OK;
+}
+static int check_cb(int cb) { return (cb == 0 || cb == 1) ? QPHIX_OK : fail(QPHIX_EDOMAIN, "checkerboard %d", cb); }
+
+extern "C" int qphix_pack_spinor(const qphix_geom *g, void *dst, const double *src, int cb) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(dst); CHKPTR(src);
+  TRY(check_cb(cb));
+  DevBuf b;
+  TRY(stage_in(b, src, (size_t)g->d.vh * kSpinorReals));
+  HIPCHK(fld_pack_spinor(g->d, dst, (const double *)b.p, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_unpack_spinor(const qphix_geom *g, double *dst, const void *src, int cb) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(dst); CHKPTR(src);
+  TRY(check_cb(cb));
+  DevBuf b;
+  const size_t n = (size_t)g->d.vh * kSpinorReals;
+  if (hipMalloc(&b.p, n * sizeof(double)) != hipSuccess) return fail(QPHIX_ENOMEM, "staging allocation failed");
+  HIPCHK(fld_unpack_spinor(g->d, (double *)b.p, src, g_stream));
+  HIPCHK(hipMemcpyAsync(dst, b.p, n * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_pack_gauge(const qphix_geom *g, void *u0, void *u1, const double *src) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(u0); CHKPTR(u1); CHKPTR(src);
+  DevBuf b;
+  TRY(stage_in(b, src, (size_t)g->d.vh * 2 * 72));
+  HIPCHK(fld_pack_gauge_cb(g->d, u0, 0, (const double *)b.p, g_stream));
+  HIPCHK(fld_pack_gauge_cb(g->d, u1, 1, (const double *)b.p, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_pack_clover(const qphix_geom *g, void *dst, const double *src, int cb) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(dst); CHKPTR(src);
+  TRY(check_cb(cb));
+  DevBuf b;
+  TRY(stage_in(b, src, (size_t)g->d.vh * kCloverReals));
+  HIPCHK(fld_pack_clover(g->d, dst, (const double *)b.p, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_unpack_clover(const qphix_geom *g, double *dst, const void *src, int cb) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(dst); CHKPTR(src);
+  TRY(check_cb(cb));
+  DevBuf b;
+  const size_t n = (size_t)g->d.vh * kCloverReals;
+  if (hipMalloc(&b.p, n * sizeof(double)) != hipSuccess) return fail(QPHIX_ENOMEM, "staging allocation failed");
+  HIPCHK(fld_unpack_clover(g->d, (double *)b.p, src, g_stream));
+  HIPCHK(hipMemcpyAsync(dst, b.p, n * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_pack_full_clover(const qphix_geom *g, void *dst, const double *src, int cb) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(dst); CHKPTR(src);
+  TRY(check_cb(cb));
+  DevBuf b;
+  TRY(stage_in(b, src, (size_t)g->d.vh * kFullCloverReals));
+  HIPCHK(fld_pack_full_clover(g->d, dst, (const double *)b.p, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+
+static int same_lattice(const qphix_geom *a, const qphix_geom *b) {
+  for (int k = 0; k < 4; k++)
+    if (a->d.L[k] != b->d.L[k]) return fail(QPHIX_EDOMAIN, "geometries describe different lattices");
+  return QPHIX_OK;
+}
+extern "C" int qphix_convert_spinor(const qphix_geom *go, void *out, double scale, const qphix_geom *gi,
+                                    const void *in) {
+  ENSURE_INIT();
+  CHKPTR(go); CHKPTR(gi); CHKPTR(out); CHKPTR(in);
+  TRY(same_lattice(go, gi));
+  HIPCHK(fld_convert_spinor(go->d, out, scale, gi->d, in, g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_convert_gauge(const qphix_geom *go, void *out, const qphix_geom *gi, const void *in) {
+  ENSURE_INIT();
+  CHKPTR(go); CHKPTR(gi); CHKPTR(out); CHKPTR(in);
+  TRY(same_lattice(go, gi));
+  HIPCHK(fld_convert_gauge(go->d, out, gi->d, in, g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_convert_clover(const qphix_geom *go, void *out, const qphix_geom *gi, const void *in, int full) {
+  ENSURE_INIT();
+  CHKPTR(go); CHKPTR(gi); CHKPTR(out); CHKPTR(in);
+  TRY(same_lattice(go, gi));
+  HIPCHK(fld_convert_clover(go->d, out, gi->d, in, full, g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_gen_gauge(const qphix_geom *g, void *u0, void *u1, uint64_t seed, double eps) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(u0); CHKPTR(u1);
+  DevBuf b;
+  const size_t n = (size_t)g->d.vh * 2 * 72;
+  if (hipMalloc(&b.p, n * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(QPHIX_ENOMEM, "gauge staging allocation failed");
+  }
+  HIPCHK(fld_gen_gauge_canonical(g->d, (double *)b.p, seed, eps, g_stream));
+  HIPCHK(fld_pack_gauge_cb(g->d, u0, 0, (const double *)b.p, g_stream));
+  HIPCHK(fld_pack_gauge_cb(g->d, u1, 1, (const double *)b.p, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_gen_spinor(const qphix_geom *g, void *dst, uint64_t seed, int cb) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(dst);
+  TRY(check_cb(cb));
+  HIPCHK(fld_gen_spinor(g->d, dst, seed, cb, g_stream));
+  return QPHIX_OK;
+}
+extern "C" int qphix_clover_build(const qphix_geom *g, const void *u0, const void *u1, double csw_r, double csw_t,
+                                  double diag_mass, int cb, int what, double mu, double eps, void *out) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(u0); CHKPTR(u1); CHKPTR(out);
+  TRY(check_cb(cb));
+  if (what < QPHIX_CLOV_A || what > QPHIX_CLOV_ND_OFF) return fail(QPHIX_EDOMAIN, "unknown clover output %d", what);
+  // clover coefficients enter halved (tests/clover_term_qdp_w.h:165-182)
+  HIPCHK(fld_clover_build(g->d, u0, u1, 0.5 * csw_r, 0.5 * csw_t, diag_mass, cb, what, mu, eps, out, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BLAS plumbing
+// ---------------------------------------------------------------------------
+static int run_blas(const qphix_geom *g, int op, const BlasArgs &a, double *out) {
+  int nb = 0;
+  HIPCHK(blas_launch(g->d, op, a, g_partials, &nb, g_stream));
+  const int nr = blas_nred(op);
+  if (nr > 0) {
+    HIPCHK(blas_finalize(g_partials, nb, nr, g_dres, g_stream));
+    HIPCHK(hipMemcpyAsync(g_hres, g_dres, nr * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (out)
+      for (int k = 0; k < nr; k++) out[k] = g_hres[k];
+  }
+  return QPHIX_OK;
+}
+static BlasArgs bargs() {
+  BlasArgs a;
+  memset(&a, 0, sizeof(a));
+  return a;
+}
+
+#define BLAS_PRE() \
+  ENSURE_INIT();   \
+  CHKPTR(g)
+
+extern "C" int qphix_copy_spinor(const qphix_geom *g, void *res, const void *src) {
+  BLAS_PRE(); CHKPTR(res); CHKPTR(src);
+  BlasArgs a = bargs(); a.x = src; a.o0 = res;
+  return run_blas(g, B_COPY, a, nullptr);
+}
+extern "C" int qphix_zero_spinor(const qphix_geom *g, void *res) {
+  BLAS_PRE(); CHKPTR(res);
+  BlasArgs a = bargs(); a.o0 = res;
+  return run_blas(g, B_ZERO, a, nullptr);
+}
+extern "C" int qphix_axy(const qphix_geom *g, double alpha, const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha;
+  return run_blas(g, B_AXY, a, nullptr);
+}
+extern "C" int qphix_ax(const qphix_geom *g, double alpha, void *x) {
+  BLAS_PRE(); CHKPTR(x);
+  BlasArgs a = bargs(); a.o0 = x; a.a[0] = alpha;
+  return run_blas(g, B_AX, a, nullptr);
+}
+extern "C" int qphix_ypeqx(const qphix_geom *g, const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y;
+  return run_blas(g, B_YPEQX, a, nullptr);
+}
+extern "C" int qphix_ymeqx(const qphix_geom *g, const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y;
+  return run_blas(g, B_YMEQX, a, nullptr);
+}
+extern "C" int qphix_aypx(const qphix_geom *g, double alpha, const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha;
+  return run_blas(g, B_AYPX, a, nullptr);
+}
+extern "C" int qphix_twisted_mass(const qphix_geom *g, const double apimu[2], const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(apimu); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = apimu[0]; a.a[1] = apimu[1];
+  return run_blas(g, B_TWM, a, nullptr);
+}
+extern "C" int qphix_two_flav_twisted_mass(const qphix_geom *g, const double apimu[2], double epsilon,
+                                           const void *const x[2], void *const y[2]) {
+  BLAS_PRE(); CHKPTR(apimu); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x[0]; a.y = x[1]; a.o0 = y[0]; a.o1 = y[1];
+  a.a[0] = apimu[0]; a.a[1] = apimu[1]; a.a[2] = epsilon;
+  return run_blas(g, B_TWM2, a, nullptr);
+}
+extern "C" int qphix_axpy(const qphix_geom *g, double alpha, const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha;
+  return run_blas(g, B_AXPY, a, nullptr);
+}
+extern "C" int qphix_axpby(const qphix_geom *g, double alpha, const void *x, double beta, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha; a.a[1] = beta;
+  return run_blas(g, B_AXPBY, a, nullptr);
+}
+extern "C" int qphix_norm2_spinor(const qphix_geom *g, const void *x, double *n2) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(n2);
+  BlasArgs a = bargs(); a.x = x;
+  return run_blas(g, B_NORM2, a, n2);
+}
+extern "C" int qphix_axpy_norm2(const qphix_geom *g, double alpha, const void *x, void *y, double *n2) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y); CHKPTR(n2);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha;
+  return run_blas(g, B_AXPYNORM2, a, n2);
+}
+extern "C" int qphix_xmy_norm2(const qphix_geom *g, void *res, const void *x, const void *y, double *n2) {
+  BLAS_PRE(); CHKPTR(res); CHKPTR(x); CHKPTR(y); CHKPTR(n2);
+  BlasArgs a = bargs(); a.x = x; a.y = y; a.o0 = res;
+  return run_blas(g, B_XMYNORM2, a, n2);
+}
+extern "C" int qphix_xmy2_norm2(const qphix_geom *g, const void *x, void *y, double *n2) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y); CHKPTR(n2);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y;
+  return run_blas(g, B_XMY2NORM2, a, n2);
+}
+extern "C" int qphix_rmammp_norm2_rxpap(const qphix_geom *g, void *r, double ar, const void *mmp, void *x,
+                                        const void *p, double *cp) {
+  BLAS_PRE(); CHKPTR(r); CHKPTR(mmp); CHKPTR(x); CHKPTR(p); CHKPTR(cp);
+  BlasArgs a = bargs(); a.x = mmp; a.y = p; a.o0 = r; a.o1 = x; a.a[0] = ar;
+  return run_blas(g, B_RMAMMP, a, cp);
+}
+extern "C" int qphix_richardson_rxupdate_norm_r(const qphix_geom *g, void *x, void *r, const void *dx,
+                                                const void *dr, double *n2) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(r); CHKPTR(dx); CHKPTR(dr); CHKPTR(n2);
+  BlasArgs a = bargs(); a.x = dx; a.y = dr; a.o0 = x; a.o1 = r;
+  return run_blas(g, B_RICH, a, n2);
+}
+extern "C" int qphix_bicgstab_xmy(const qphix_geom *g, const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y;
+  return run_blas(g, B_BICG_XMY, a, nullptr);
+}
+extern "C" int qphix_inner_product(const qphix_geom *g, const void *l, const void *r, double out[2]) {
+  BLAS_PRE(); CHKPTR(l); CHKPTR(r); CHKPTR(out);
+  BlasArgs a = bargs(); a.x = l; a.y = r;
+  return run_blas(g, B_CDOT, a, out);
+}
+extern "C" int qphix_inner_product_norm(const qphix_geom *g, const void *l, const void *r, double out[3]) {
+  BLAS_PRE(); CHKPTR(l); CHKPTR(r); CHKPTR(out);
+  BlasArgs a = bargs(); a.x = l; a.y = r;
+  return run_blas(g, B_CDOTNORM, a, out);
+}
+extern "C" int qphix_bicgstab_p_update(const qphix_geom *g, const void *r, void *p, const void *v,
+                                       const double beta[2], const double omega[2]) {
+  BLAS_PRE(); CHKPTR(r); CHKPTR(p); CHKPTR(v); CHKPTR(beta); CHKPTR(omega);
+  BlasArgs a = bargs(); a.x = r; a.y = v; a.o0 = p;
+  a.a[0] = beta[0]; a.a[1] = beta[1]; a.a[2] = omega[0]; a.a[3] = omega[1];
+  return run_blas(g, B_PUPD, a, nullptr);
+}
+extern "C" int qphix_bicgstab_s_update(const qphix_geom *g, const double alpha[2], void *s, const void *v) {
+  BLAS_PRE(); CHKPTR(alpha); CHKPTR(s); CHKPTR(v);
+  BlasArgs a = bargs(); a.x = v; a.o0 = s; a.a[0] = alpha[0]; a.a[1] = alpha[1];
+  return run_blas(g, B_SUPD, a, nullptr);
+}
+extern "C" int qphix_caxpy(const qphix_geom *g, const double alpha[2], const void *x, void *y) {
+  BLAS_PRE(); CHKPTR(alpha); CHKPTR(x); CHKPTR(y);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha[0]; a.a[1] = alpha[1];
+  return run_blas(g, B_CAXPY, a, nullptr);
+}
+extern "C" int qphix_bicgstab_rxupdate(const qphix_geom *g, void *x, void *r, const void *t, const void *p,
+                                       const double omega[2], const double alpha[2], double *r_norm) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(r); CHKPTR(t); CHKPTR(p); CHKPTR(omega); CHKPTR(alpha); CHKPTR(r_norm);
+  BlasArgs a = bargs(); a.x = t; a.y = p; a.o0 = x; a.o1 = r;
+  a.a[0] = omega[0]; a.a[1] = omega[1]; a.a[2] = alpha[0]; a.a[3] = alpha[1];
+  return run_blas(g, B_RXUPD, a, r_norm);
+}
+extern "C" int qphix_mr_rxupdate(const qphix_geom *g, void *x, void *r, const void *mr, const double ac[2]) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(r); CHKPTR(mr); CHKPTR(ac);
+  BlasArgs a = bargs(); a.x = mr; a.o0 = x; a.o1 = r; a.a[0] = ac[0]; a.a[1] = ac[1];
+  return run_blas(g, B_MRUPD, a, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Dslash objects
+// ---------------------------------------------------------------------------
+extern "C" int qphix_dslash_create(const qphix_geom *g, const qphix_dslash_params *p, qphix_dslash **d) {
+  CHKPTR(g); CHKPTR(p); CHKPTR(d);
+  qphix_dslash *x = new qphix_dslash;
+  x->g = g;
+  x->p = *p;
+  *d = x;
+  return QPHIX_OK;
+}
+extern "C" int qphix_dslash_destroy(qphix_dslash *d) {
+  delete d;
+  return QPHIX_OK;
+}
+
+static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *res, const void *psi, const void *u,
+                      const void *clov, const void *chi, double coef, double alpha, double mu, double muinv,
+                      double rho) {
+  ENSURE_INIT();
+  CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(u);
+  if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1, got %d", isign);
+  TRY(check_cb(cb));
+  if ((kind == KD_CLOVER || kind == KA_CLOVER || kind == KD_FULL) && !clov)
+    return fail(QPHIX_EINVAL, "clover field required");
+  if ((kind == KA_WILSON || kind == KA_CLOVER || kind == KA_TM) && !chi) return fail(QPHIX_EINVAL, "chi required");
+  const GeomDesc &g = d->g->d;
+  HopArgs a;
+  memset(&a, 0, sizeof(a));
+  a.psi = psi;
+  a.out = res;
+  a.u = u;
+  a.clov = clov;
+  a.chi = chi;
+  a.vs = g.vs;
+  a.vh = (int)g.vh;
+  a.nxh = g.L[0] / 2;
+  a.ny = g.L[1];
+  a.nz = g.L[2];
+  a.nt = g.L[3];
+  a.cb = cb;
+  a.cs = coef * d->p.aniso_s;
+  a.ct = coef * d->p.aniso_t;
+  a.ctf = a.ct * d->p.t_boundary;
+  a.ctb = a.ct * d->p.t_boundary;
+  for (int k = 0; k < 4; k++) {
+    if (d->p.use_tbc[k]) a.tbc_mask |= 1 << k;
+    a.tbc_r[k] = d->p.tbc_phases[k][0];
+    a.tbc_i[k] = d->p.tbc_phases[k][1];
+  }
+  a.alpha = alpha;
+  a.mu = mu;
+  a.muinv = muinv;
+  a.rho = rho;
+  hipError_t e;
+  switch (g.prec) {
+    case P64: e = launch_hop_prec<P64>(kind, g.c12 != 0, isign, a, g_hop_block, g_stream); break;
+    case P32: e = launch_hop_prec<P32>(kind, g.c12 != 0, isign, a, g_hop_block, g_stream); break;
+    default: e = launch_hop_prec<P16>(kind, g.c12 != 0, isign, a, g_hop_block, g_stream); break;
+  }
+  if (e != hipSuccess) return fail(QPHIX_EHIP, "hop kernel launch: %s", hipGetErrorString(e));
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_wilson_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign, int cb) {
+  return launch_hop(d, KD_WILSON, isign, cb, res, psi, u, nullptr, nullptr, 1.0, 0, 0, 0, 0);
+}
+extern "C" int qphix_wilson_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi, const void *u,
+                                       double alpha, double beta, int isign, int cb) {
+  return launch_hop(d, KA_WILSON, isign, cb, res, psi, u, nullptr, chi, -beta, alpha, 0, 0, 0);
+}
+extern "C" int qphix_clov_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, const void *invclov,
+                                 int isign, int cb) {
+  return launch_hop(d, KD_CLOVER, isign, cb, res, psi, u, invclov, nullptr, 1.0, 0, 0, 0, 0);
+}
+extern "C" int qphix_clov_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi, const void *u,
+                                     const void *clov, double beta, int isign, int cb) {
+  CHKPTR(d);
+  return launch_hop(d, KA_CLOVER, isign, cb, res, psi, u, clov, chi, -beta, 0, 0, 0, d->p.prec_mass_rho);
+}
+extern "C" int qphix_clov_mult(qphix_dslash *d, void *res, const void *chi, const void *clov) {
+  ENSURE_INIT();
+  CHKPTR(d); CHKPTR(res); CHKPTR(chi); CHKPTR(clov);
+  BlasArgs a = bargs(); a.x = chi; a.y = clov; a.o0 = res;
+  return run_blas(d->g, B_CLOVMULT, a, nullptr);
+}
+extern "C" int qphix_tm_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign, int cb) {
+  CHKPTR(d);
+  const double al = 4.0 + d->p.mass, mu = d->p.twisted_mass;
+  return launch_hop(d, KD_TM, isign, cb, res, psi, u, nullptr, nullptr, 1.0, 0, mu / al, al / (mu * mu + al * al), 0);
+}
+extern "C" int qphix_tm_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi, const void *u,
+                                   double alpha, double beta, int isign, int cb, double musign) {
+  CHKPTR(d);
+  const double al = 4.0 + d->p.mass;
+  return launch_hop(d, KA_TM, isign, cb, res, psi, u, nullptr, chi, -beta, alpha, musign * d->p.twisted_mass / al, 0,
+                    0);
+}
+extern "C" int qphix_tm_two_flav_achimbdpsi(qphix_dslash *d, void *const res[2], const void *const psi[2],
+                                            const void *const chi[2], const void *u, double alpha, double beta,
+                                            double epsilon, int isign, int cb) {
+  CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(chi);
+  for (int f = 0; f < 2; f++) {
+    TRY(qphix_tm_achimbdpsi(d, res[f], psi[f], chi[f], u, alpha, beta, isign, cb, 1.0 - 2.0 * f));
+    TRY(qphix_axpy(d->g, -epsilon, chi[1 - f], res[f]));
+  }
+  return QPHIX_OK;
+}
+extern "C" int qphix_tmclov_dslash(qphix_dslash *d, void *res, const void *psi, const void *u,
+                                   const void *const invclov[2], int isign, int cb, int fl) {
+  CHKPTR(invclov);
+  // up flavour: +mu block for M, -mu block for M^+; reversed for down
+  const int clidx = (fl == 0) ? (isign == 1 ? 0 : 1) : (isign == 1 ? 1 : 0);
+  return launch_hop(d, KD_FULL, isign, cb, res, psi, u, invclov[clidx], nullptr, 1.0, 0, 0, 0, 0);
+}
+extern "C" int qphix_tmclov_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi, const void *u,
+                                       const void *clov, double beta, int isign, int cb, int fl) {
+  CHKPTR(d);
+  const double rho = (fl == 1 ? -1.0 : 1.0) * d->p.prec_mass_rho;
+  return launch_hop(d, KA_CLOVER, isign, cb, res, psi, u, clov, chi, -beta, 0, 0, 0, rho);
+}
+extern "C" int qphix_tmclov_two_flav_achimbdpsi(qphix_dslash *d, void *const res[2], const void *const psi[2],
+                                                const void *const chi[2], const void *u, const void *clov,
+                                                double beta, double epsilon, int isign, int cb) {
+  CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(chi);
+  for (int f = 0; f < 2; f++) {
+    TRY(qphix_tmclov_achimbdpsi(d, res[f], psi[f], chi[f], u, clov, beta, isign, cb, f));
+    TRY(qphix_axpy(d->g, -epsilon, chi[1 - f], res[f]));
+  }
+  return QPHIX_OK;
+}
+extern "C" int qphix_tmclov_two_flav_inverse_clover_term(qphix_dslash *d, void *const res[2],
+                                                         const void *const psi[2], const void *const fcl[2],
+                                                         const void *cl_offdiag, int isign) {
+  ENSURE_INIT();
+  CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(fcl); CHKPTR(cl_offdiag);
+  BlasArgs a = bargs();
+  a.x = psi[0];
+  a.y = psi[1];
+  a.z = isign == 1 ? fcl[0] : fcl[1];
+  a.w = isign == 1 ? fcl[1] : fcl[0];
+  a.o2 = const_cast<void *>(cl_offdiag);
+  a.o0 = res[0];
+  a.o1 = res[1];
+  return run_blas(d->g, B_TWOFLAV_INVCLOV, a, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// even-odd operators
+// ---------------------------------------------------------------------------
+extern "C" int qphix_op_create(const qphix_geom *g, const qphix_op_params *p, qphix_op **out) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(p); CHKPTR(out);
+  if (p->kind < QPHIX_OP_WILSON || p->kind > QPHIX_OP_NDTMCLOVER)
+    return fail(QPHIX_EDOMAIN, "unknown operator kind %d", p->kind);
+  qphix_op *op = new qphix_op;
+  memset(op, 0, sizeof(*op));
+  op->g = g;
+  op->p = *p;
+  op->d.g = g;
+  op->d.p = p->hop;
+  op->d.p.mass = p->mass;
+  op->d.p.twisted_mass = p->twisted_mass;
+  op->d.p.prec_mass_rho = (p->kind == QPHIX_OP_NDTMCLOVER) ? p->twisted_mass : p->prec_mass_rho;
+  op->nflav = (p->kind == QPHIX_OP_NDTMWILSON || p->kind == QPHIX_OP_NDTMCLOVER) ? 2 : 1;
+  for (int f = 0; f < op->nflav; f++) {
+    int r = qphix_alloc_spinor(g, &op->tmp[f]);
+    if (!r && op->nflav == 2) r = qphix_alloc_spinor(g, &op->tmp2[f]);
+    if (r) {
+      qphix_op_destroy(op);
+      return r;
+    }
+  }
+  *out = op;
+  return QPHIX_OK;
+}
+extern "C" int qphix_op_destroy(qphix_op *op) {
+  if (!op) return QPHIX_OK;
+  for (int f = 0; f < 2; f++) {
+    qphix_free(op->tmp[f]);
+    qphix_free(op->tmp2[f]);
+  }
+  delete op;
+  return QPHIX_OK;
+}
+extern "C" int qphix_op_nflav(const qphix_op *op) { return op ? op->nflav : 0; }
+
+extern "C" int qphix_op_apply(qphix_op *op, void *const res[], const void *const in[], int isign, int t) {
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  TRY(check_cb(t));
+  const int o = 1 - t;
+  const qphix_op_params &p = op->p;
+  qphix_dslash *D = &op->d;
+  switch (p.kind) {
+    case QPHIX_OP_WILSON: {
+      const double al = 4.0 + p.mass;
+      TRY(qphix_wilson_dslash(D, op->tmp[0], in[0], p.u[o], isign, o));
+      return qphix_wilson_achimbdpsi(D, res[0], op->tmp[0], in[0], p.u[t], al, 0.25 / al, isign, t);
+    }
+    case QPHIX_OP_CLOVER:
+      TRY(qphix_clov_dslash(D, op->tmp[0], in[0], p.u[o], p.invclov[o], isign, o));
+      return qphix_clov_achimbdpsi(D, res[0], op->tmp[0], in[0], p.u[t], p.clov[t], 0.25, isign, t);
+    case QPHIX_OP_TMWILSON:
+      TRY(qphix_tm_dslash(D, op->tmp[0], in[0], p.u[o], isign, o));
+      return qphix_tm_achimbdpsi(D, res[0], op->tmp[0], in[0], p.u[t], 4.0 + p.mass, 0.25, isign, t, 1.0);
+    case QPHIX_OP_TMCLOVER:
+      TRY(qphix_tmclov_dslash(D, op->tmp[0], in[0], p.u[o], p.fullclov[o], isign, o, 0));
+      return qphix_tmclov_achimbdpsi(D, res[0], op->tmp[0], in[0], p.u[t], p.clov[t], 0.25, isign, t, 0);
+    case QPHIX_OP_NDTMWILSON: {
+      const double al = 4.0 + p.mass, mu = p.twisted_mass, eps = p.epsilon;
+      const double mi = 1.0 / (al * al + mu * mu - eps * eps);
+      for (int f = 0; f < 2; f++) TRY(qphix_wilson_dslash(D, op->tmp[f], in[f], p.u[o], isign, o));
+      const double apimu[2] = {al * mi, -isign * mu * mi};
+      TRY(qphix_two_flav_twisted_mass(op->g, apimu, eps * mi, op->tmp, op->tmp2));
+      return qphix_tm_two_flav_achimbdpsi(D, res, op->tmp2, in, p.u[t], al, 0.25, eps, isign, t);
+    }
+    case QPHIX_OP_NDTMCLOVER: {
+      for (int f = 0; f < 2; f++) TRY(qphix_wilson_dslash(D, op->tmp[f], in[f], p.u[o], isign, o));
+      TRY(qphix_tmclov_two_flav_inverse_clover_term(D, op->tmp2, op->tmp, p.fullclov[o], p.clov_odiag[o], isign));
+      return qphix_tmclov_two_flav_achimbdpsi(D, res, op->tmp2, in, p.u[t], p.clov[t], 0.25, p.epsilon, isign, t);
+    }
+  }
+  return fail(QPHIX_EDOMAIN, "unknown operator kind");
+}
+
+// ---------------------------------------------------------------------------
+// solvers
+// ---------------------------------------------------------------------------
+typedef std::vector<void *> SV;  // one spinor per flavour
+
+static int alloc_set(const qphix_geom *g, int nf, SV &v, qphix_solver *s) {
+  v.assign(nf, nullptr);
+  for (int f = 0; f < nf; f++) {
+    TRY(qphix_alloc_spinor(g, &v[f]));
+    s->work.push_back(v[f]);
+  }
+  return QPHIX_OK;
+}
+
+// multi-flavour BLAS helpers (reductions summed over flavours)
+static int mf_norm2(const qphix_geom *g, int nf, const void *const *x, double &n) {
+  n = 0;
+  for (int f = 0; f < nf; f++) {
+    double t;
+    TRY(qphix_norm2_spinor(g, x[f], &t));
+    n += t;
+  }
+  return QPHIX_OK;
+}
+static int mf_cdot(const qphix_geom *g, int nf, const void *const *l, const void *const *r, double o[2]) {
+  o[0] = o[1] = 0;
+  for (int f = 0; f < nf; f++) {
+    double t[2];
+    TRY(qphix_inner_product(g, l[f], r[f], t));
+    o[0] += t[0];
+    o[1] += t[1];
+  }
+  return QPHIX_OK;
+}
+static int op_apply_v(qphix_op *M, SV &res, const void *const *in, int isign, int cb) {
+  return qphix_op_apply(M, res.data(), in, isign, cb);
+}
+static const void *const *cv(const SV &v) { return (const void *const *)v.data(); }
+
+static void cdiv(double *o, const double *a, const double *b) {
+  const double den = b[0] * b[0] + b[1] * b[1];
+  const double re = (a[0] * b[0] + a[1] * b[1]) / den, im = (a[1] * b[0] - a[0] * b[1]) / den;
+  o[0] = re;
+  o[1] = im;
+}
+static void cmul(double *o, const double *a, const double *b) {
+  const double re = a[0] * b[0] - a[1] * b[1], im = a[0] * b[1] + a[1] * b[0];
+  o[0] = re;
+  o[1] = im;
+}
+
+extern "C" int qphix_cg_create(qphix_op *M, int max_iters, qphix_solver **s) {
+  CHKPTR(M); CHKPTR(s);
+  qphix_solver *x = new qphix_solver;
+  x->kind = S_CG;
+  x->M = M;
+  x->max_iters = max_iters;
+  x->inner = nullptr;
+  x->delta = 0;
+  x->mdagm = 0;
+  x->max_shifts = 0;
+  *s = x;
+  return QPHIX_OK;
+}
+extern "C" int qphix_bicgstab_create(qphix_op *M, int max_iters, qphix_solver **s) {
+  TRY(qphix_cg_create(M, max_iters, s));
+  (*s)->kind = S_BICGSTAB;
+  return QPHIX_OK;
+}
+extern "C" int qphix_richardson_create(qphix_op *M, qphix_solver *inner, double delta, int max_iters, int mdagm,
+                                       qphix_solver **s) {
+  CHKPTR(inner);
+  TRY(qphix_cg_create(M, max_iters, s));
+  (*s)->kind = S_RICHARDSON;
+  (*s)->inner = inner;
+  (*s)->delta = delta;
+  (*s)->mdagm = mdagm;
+  if (inner->M->nflav != M->nflav) return fail(QPHIX_EDOMAIN, "inner and outer operators differ in flavours");
+  return QPHIX_OK;
+}
+extern "C" int qphix_minvcg_create(qphix_op *M, int max_iters, int max_shifts, qphix_solver **s) {
+  TRY(qphix_cg_create(M, max_iters, s));
+  (*s)->kind = S_MINVCG;
+  (*s)->max_shifts = max_shifts;
+  return QPHIX_OK;
+}
+extern "C" int qphix_solver_destroy(qphix_solver *s) {
+  if (!s) return QPHIX_OK;
+  for (void *p : s->work) qphix_free(p);
+  delete s;
+  return QPHIX_OK;
+}
+
+// invcg.h:88-456
+static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
+                    int verbose, int cb, int rtype, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  const int nf = M->nflav;
+  SV mp, mmp, p, r;
+  if (s->work.empty()) {
+    TRY(alloc_set(g, nf, mp, s));
+    TRY(alloc_set(g, nf, mmp, s));
+    TRY(alloc_set(g, nf, p, s));
+    TRY(alloc_set(g, nf, r, s));
+  } else {
+    mp.assign(s->work.begin(), s->work.begin() + nf);
+    mmp.assign(s->work.begin() + nf, s->work.begin() + 2 * nf);
+    p.assign(s->work.begin() + 2 * nf, s->work.begin() + 3 * nf);
+    r.assign(s->work.begin() + 3 * nf, s->work.begin() + 4 * nf);
+  }
+  st->site_flops = 0;
+  st->mv_apps = 0;
+  st->status = QPHIX_OK;
+  double chi_sq;
+  TRY(mf_norm2(g, nf, rhs, chi_sq));
+  double rsd_sq = rsd_target * rsd_target;
+  if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= chi_sq;
+  TRY(op_apply_v(M, mp, x, isign, cb));
+  TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+  st->mv_apps += 2;
+  double cp = 0;
+  for (int f = 0; f < nf; f++) {
+    double t;
+    TRY(qphix_xmy_norm2(g, r[f], rhs[f], mmp[f], &t));
+    cp += t;
+  }
+  if (verbose) printf("CG: ||r||^2 = %e target = %e\n", cp, rsd_sq);
+  if (cp <= rsd_sq) {
+    st->n_iters = 0;
+    st->rsd_sq_final = cp / chi_sq;
+    return QPHIX_OK;
+  }
+  for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, p[f], r[f]));
+  for (int k = 0; k <= s->max_iters; ++k) {
+    const double c = cp;
+    TRY(op_apply_v(M, mp, cv(p), isign, cb));
+    TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+    st->mv_apps += 2;
+    double d;
+    TRY(mf_norm2(g, nf, cv(mp), d));
+    const double a = c / d;
+    cp = 0;
+    for (int f = 0; f < nf; f++) {
+      double t;
+      TRY(qphix_rmammp_norm2_rxpap(g, r[f], a, mmp[f], x[f], p[f], &t));
+      cp += t;
+    }
+    if (verbose) printf("CG: iter %d: r = %e target = %e\n", k, cp, rsd_sq);
+    if (cp < rsd_sq) {
+      st->n_iters = k;
+      TRY(op_apply_v(M, mp, x, isign, cb));
+      TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+      st->mv_apps += 2;
+      cp = 0;
+      for (int f = 0; f < nf; f++) {
+        double t;
+        TRY(qphix_xmy_norm2(g, r[f], rhs[f], mmp[f], &t));
+        cp += t;
+      }
+      st->rsd_sq_final = cp;
+      return QPHIX_OK;
+    }
+    const double b = cp / c;
+    for (int f = 0; f < nf; f++) TRY(qphix_aypx(g, b, r[f], p[f]));
+  }
+  st->n_iters = s->max_iters;
+  st->rsd_sq_final = cp;
+  st->status = QPHIX_ENOCONV;
+  return QPHIX_OK;
+}
+
+// invbicgstab.h:86-262
+static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
+                          int verbose, int cb, int rtype, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  const int nf = M->nflav;
+  SV r0, r, p, v, t;
+  if (s->work.empty()) {
+    TRY(alloc_set(g, nf, r0, s));
+    TRY(alloc_set(g, nf, r, s));
+    TRY(alloc_set(g, nf, p, s));
+    TRY(alloc_set(g, nf, v, s));
+    TRY(alloc_set(g, nf, t, s));
+  } else {
+    SV *sets[5] = {&r0, &r, &p, &v, &t};
+    for (int k = 0; k < 5; k++) sets[k]->assign(s->work.begin() + k * nf, s->work.begin() + (k + 1) * nf);
+  }
+  st->site_flops = 0;
+  st->mv_apps = 0;
+  st->status = QPHIX_OK;
+  double rhs_sq;
+  TRY(mf_norm2(g, nf, rhs, rhs_sq));
+  double rsd_sq = rsd_target * rsd_target;
+  if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= rhs_sq;
+  TRY(op_apply_v(M, r0, x, isign, cb));
+  st->mv_apps++;
+  for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r0[f]));
+  double r_norm;
+  TRY(mf_norm2(g, nf, cv(r0), r_norm));
+  if (r_norm < rsd_sq) {
+    st->rsd_sq_final = r_norm;
+    st->n_iters = 0;
+    return QPHIX_OK;
+  }
+  for (int f = 0; f < nf; f++) {
+    TRY(qphix_copy_spinor(g, r[f], r0[f]));
+    TRY(qphix_zero_spinor(g, p[f]));
+    TRY(qphix_zero_spinor(g, v[f]));
+  }
+  double rho_prev[2] = {1, 0}, alpha[2] = {1, 0}, omega[2] = {1, 0}, rho[2] = {0, 0};
+  bool notconv = true;
+  int k;
+  for (k = 1; k <= s->max_iters && notconv; k++) {
+    TRY(mf_cdot(g, nf, cv(r0), cv(r), rho));
+    if (rho[0] == 0 && rho[1] == 0) {
+      st->rsd_sq_final = r_norm;
+      st->n_iters = k;
+      st->status = QPHIX_EBREAKDOWN;
+      return QPHIX_OK;
+    }
+    double t1[2], t2[2], beta[2];
+    cdiv(t1, rho, rho_prev);
+    cdiv(t2, alpha, omega);
+    cmul(beta, t1, t2);
+    for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_p_update(g, r[f], p[f], v[f], beta, omega));
+    TRY(op_apply_v(M, v, cv(p), isign, cb));
+    st->mv_apps++;
+    double tmp[2];
+    TRY(mf_cdot(g, nf, cv(r0), cv(v), tmp));
+    if (tmp[0] == 0 && tmp[1] == 0) {
+      st->n_iters = k;
+      st->rsd_sq_final = r_norm;
+      st->status = QPHIX_EBREAKDOWN;
+      return QPHIX_OK;
+    }
+    cdiv(alpha, rho, tmp);
+    rho_prev[0] = rho[0];
+    rho_prev[1] = rho[1];
+    for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_s_update(g, alpha, r[f], v[f]));
+    TRY(op_apply_v(M, t, cv(r), isign, cb));
+    st->mv_apps++;
+    double t_norm;
+    TRY(mf_norm2(g, nf, cv(t), t_norm));
+    if (t_norm == 0) {
+      st->rsd_sq_final = r_norm;
+      st->n_iters = k;
+      st->status = QPHIX_EBREAKDOWN;
+      return QPHIX_OK;
+    }
+    TRY(mf_cdot(g, nf, cv(t), cv(r), omega));
+    omega[0] /= t_norm;
+    omega[1] /= t_norm;
+    r_norm = 0;
+    for (int f = 0; f < nf; f++) {
+      double rn;
+      TRY(qphix_bicgstab_rxupdate(g, x[f], r[f], t[f], p[f], omega, alpha, &rn));
+      r_norm += rn;
+    }
+    if (verbose) printf("BICGSTAB: iter %d r_norm = %e target = %e\n", k, r_norm, rsd_sq);
+    if (r_norm < rsd_sq) notconv = false;
+  }
+  st->rsd_sq_final = r_norm;
+  st->n_iters = k;
+  if (notconv) st->status = QPHIX_ENOCONV;
+  return QPHIX_OK;
+}
+
+// inv_richardson_multiprec.h:42-239
+static int solve_richardson(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
+                            int verbose, int cb, int rtype, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  const qphix_geom *gi = s->inner->M->g;
+  const int nf = M->nflav;
+  SV r, dx, tmp, tmp2, ri, dxi;
+  if (s->work.empty()) {
+    TRY(alloc_set(g, nf, r, s));
+    TRY(alloc_set(g, nf, dx, s));
+    TRY(alloc_set(g, nf, tmp, s));
+    TRY(alloc_set(g, nf, tmp2, s));
+    TRY(alloc_set(gi, nf, ri, s));
+    TRY(alloc_set(gi, nf, dxi, s));
+  } else {
+    SV *sets[6] = {&r, &dx, &tmp, &tmp2, &ri, &dxi};
+    for (int k = 0; k < 6; k++) sets[k]->assign(s->work.begin() + k * nf, s->work.begin() + (k + 1) * nf);
+  }
+  unsigned long mv_outer = 0, mv_inner = 0, fl_inner = 0;
+  st->status = QPHIX_OK;
+  double rhs_sq, r_norm_sq;
+  TRY(mf_norm2(g, nf, rhs, rhs_sq));
+  double rsd_t = rsd_target * rsd_target;
+  if (rtype == QPHIX_RESID_RELATIVE) rsd_t *= rhs_sq;
+  auto apply_outer = [&](const void *const *in) -> int {
+    if (s->mdagm) {
+      TRY(op_apply_v(M, tmp2, in, isign, cb));
+      TRY(op_apply_v(M, tmp, cv(tmp2), -isign, cb));
+      mv_outer += 2;
+    } else {
+      TRY(op_apply_v(M, tmp, in, isign, cb));
+      mv_outer++;
+    }
+    return QPHIX_OK;
+  };
+  auto true_resid = [&]() -> int {
+    TRY(apply_outer(x));
+    r_norm_sq = 0;
+    for (int f = 0; f < nf; f++) {
+      double t;
+      TRY(qphix_xmy_norm2(g, r[f], rhs[f], tmp[f], &t));
+      r_norm_sq += t;
+    }
+    return QPHIX_OK;
+  };
+  TRY(true_resid());
+  if (verbose) printf("RICHARDSON: Initial || r ||^2 = %16.8e Target || r ||^2 = %16.8e\n", r_norm_sq, rsd_t);
+  if (r_norm_sq < rsd_t) {
+    st->n_iters = 0;
+    st->rsd_sq_final = r_norm_sq;
+    st->mv_apps = mv_outer;
+    st->site_flops = 0;
+    return QPHIX_OK;
+  }
+  int iter;
+  for (iter = 1; iter < s->max_iters; iter++) {
+    const double delta_finish = sqrt(rsd_t / r_norm_sq);
+    const double delta_use = delta_finish > s->delta ? delta_finish : s->delta;
+    const double r_norm = sqrt(r_norm_sq);
+    for (int f = 0; f < nf; f++) {
+      TRY(qphix_convert_spinor(gi, ri[f], 1.0 / r_norm, g, r[f]));
+      TRY(qphix_zero_spinor(gi, dxi[f]));
+    }
+    qphix_solve_stats ist;
+    TRY(qphix_solve(s->inner, dxi.data(), cv(ri), delta_use, isign, 0, cb, QPHIX_RESID_RELATIVE, &ist));
+    mv_inner += ist.mv_apps;
+    fl_inner += ist.site_flops;
+    for (int f = 0; f < nf; f++) TRY(qphix_convert_spinor(g, dx[f], r_norm, gi, dxi[f]));
+    TRY(apply_outer(cv(dx)));
+    r_norm_sq = 0;
+    for (int f = 0; f < nf; f++) {
+      double t;
+      TRY(qphix_richardson_rxupdate_norm_r(g, x[f], r[f], dx[f], tmp[f], &t));
+      r_norm_sq += t;
+    }
+    if (verbose)
+      printf("RICHARDSON: Iter %d inner iters %d r_norm_sq = %e Target = %e\n", iter, ist.n_iters, r_norm_sq, rsd_t);
+    if (r_norm_sq < rsd_t) {
+      TRY(true_resid());
+      st->rsd_sq_final = r_norm_sq / rhs_sq;
+      st->n_iters = iter;
+      st->mv_apps = mv_outer + mv_inner;
+      st->site_flops = fl_inner * nf;
+      return QPHIX_OK;
+    }
+  }
+  TRY(true_resid());
+  st->rsd_sq_final = r_norm_sq / rhs_sq;
+  st->n_iters = iter;
+  st->mv_apps = mv_outer + mv_inner;
+  st->site_flops = fl_inner * nf;
+  st->status = QPHIX_ENOCONV;
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_solve(qphix_solver *s, void *const x[], const void *const rhs[], double rsd_target, int isign,
+                           int verbose, int target_cb, int resid_type, qphix_solve_stats *st) {
+  ENSURE_INIT();
+  CHKPTR(s); CHKPTR(x); CHKPTR(rhs); CHKPTR(st);
+  TRY(check_cb(target_cb));
+  if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1");
+  memset(st, 0, sizeof(*st));
+  switch (s->kind) {
+    case S_CG: return solve_cg(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
+    case S_BICGSTAB: return solve_bicgstab(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
+    case S_RICHARDSON: return solve_richardson(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
+  }
+  return fail(QPHIX_EINVAL, "qphix_solve on a multishift solver; use qphix_minvcg_solve");
+}
+
+// minvcg.h:77-387
+extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[], int n_shift,
+                                  const double *shifts, const double *rsd_target, int isign, int verbose, int cb,
+                                  int *n_iters, double *rsd_sq_final, unsigned long *site_flops,
+                                  unsigned long *mv_apps) {
+  ENSURE_INIT();
+  CHKPTR(s); CHKPTR(x); CHKPTR(rhs); CHKPTR(shifts); CHKPTR(rsd_target); CHKPTR(n_iters); CHKPTR(rsd_sq_final);
+  if (s->kind != S_MINVCG) return fail(QPHIX_EINVAL, "not a multishift solver");
+  if (n_shift < 1 || n_shift > s->max_shifts)
+    return fail(QPHIX_EDOMAIN, "n_shift = %d exceeds MaxShifts = %d", n_shift, s->max_shifts);
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  const int nf = M->nflav;
+  SV r, p0, mp, mmp;
+  std::vector<SV> p(n_shift);
+  for (void *w : s->work) qphix_free(w);
+  s->work.clear();
+  TRY(alloc_set(g, nf, r, s));
+  TRY(alloc_set(g, nf, p0, s));
+  TRY(alloc_set(g, nf, mp, s));
+  TRY(alloc_set(g, nf, mmp, s));
+  for (int q = 0; q < n_shift; q++) TRY(alloc_set(g, nf, p[q], s));
+  std::vector<double> zeta(n_shift), zeta_prev(n_shift), beta(n_shift), rsd_sq(n_shift);
+  std::vector<char> conv(n_shift, 0);
+  unsigned long mv = 0;
+  double cp;
+  TRY(mf_norm2(g, nf, rhs, cp));
+  for (int q = 0; q < n_shift; q++) rsd_sq[q] = rsd_target[q] * rsd_target[q] * cp;
+  for (int f = 0; f < nf; f++) {
+    TRY(qphix_copy_spinor(g, r[f], rhs[f]));
+    TRY(qphix_copy_spinor(g, p0[f], rhs[f]));
+    for (int q = 0; q < n_shift; q++) TRY(qphix_copy_spinor(g, p[q][f], rhs[f]));
+  }
+  TRY(op_apply_v(M, mp, cv(p0), isign, cb));
+  TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+  mv += 2;
+  double d;
+  TRY(mf_norm2(g, nf, cv(mp), d));
+  double b = -cp / d, c = 0;
+  for (int f = 0; f < nf; f++) {
+    double t;
+    TRY(qphix_axpy_norm2(g, b, mmp[f], r[f], &t));
+    c += t;
+  }
+  for (int q = 0; q < n_shift; q++) {
+    zeta_prev[q] = 1;
+    zeta[q] = 1.0 / (1.0 - shifts[q] * b);
+    beta[q] = b / (1.0 - shifts[q] * b);
+    for (int f = 0; f < nf; f++) TRY(qphix_axy(g, -beta[q], rhs[f], x[q * nf + f]));
+  }
+  bool convP = false;
+  int k;
+  *n_iters = 0;
+  for (k = 1; k < s->max_iters && !convP; ++k) {
+    const double a = c / cp;
+    for (int f = 0; f < nf; f++) TRY(qphix_aypx(g, a, r[f], p0[f]));
+    for (int q = 0; q < n_shift; q++)
+      if (!conv[q]) {
+        const double as = a * zeta[q] * beta[q] / (zeta_prev[q] * b);
+        for (int f = 0; f < nf; f++) TRY(qphix_axpby(g, zeta[q], r[f], as, p[q][f]));
+      }
+    cp = c;
+    const double bp = b;
+    TRY(op_apply_v(M, mp, cv(p0), isign, cb));
+    TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+    mv += 2;
+    TRY(mf_norm2(g, nf, cv(mp), d));
+    b = -cp / d;
+    c = 0;
+    for (int f = 0; f < nf; f++) {
+      double t;
+      TRY(qphix_axpy_norm2(g, b, mmp[f], r[f], &t));
+      c += t;
+    }
+    convP = true;
+    for (int q = 0; q < n_shift; q++) {
+      if (!conv[q]) {
+        const double z0 = zeta[q], z1 = zeta_prev[q];
+        zeta_prev[q] = z0;
+        zeta[q] = z0 * z1 * bp;
+        zeta[q] /= (b * a * (z1 - z0) + z1 * bp * (1.0 - shifts[q] * b));
+        beta[q] = b * zeta[q] / z0;
+        for (int f = 0; f < nf; f++) TRY(qphix_axpy(g, -beta[q], p[q][f], x[q * nf + f]));
+        const double css = c * zeta[q] * zeta[q];
+        if (verbose) printf("MINVCG: iter=%d shift=%d css=%16.8e target=%16.8e\n", k, q, css, rsd_sq[q]);
+        conv[q] = css < rsd_sq[q];
+        if (conv[q]) rsd_sq_final[q] = css;
+      }
+      convP &= (conv[q] != 0);
+    }
+    *n_iters = k;
+  }
+  if (site_flops) *site_flops = 0;
+  if (mv_apps) *mv_apps = mv;
+  for (void *w : s->work) qphix_free(w);
+  s->work.clear();
+  return convP ? QPHIX_OK : fail(QPHIX_ENOCONV, "multishift CG did not converge in %d iterations", k);
+}

@@ -1,0 +1,343 @@
+// Streaming BLAS over checkerboard spinors with fused FP64 reductions
+// (include/qphix/blas_new_c.h, real_functors.h, complex_functors.h).
+//
+// Every operation is one grid-stride sweep: each work-item loads its sites'
+// operands, updates them in the geometry's arithmetic type (float for F32 and
+// F16 storage, as ArithType<half> = float in the reference) and accumulates up
+// to three reductions in double. Reductions are combined by a wavefront
+// shuffle tree, then across the waves of the block through LDS, and the
+// per-block partials are summed in a fixed order by a one-block finalize
+// kernel, so results are bitwise reproducible for a given lattice.
+#include "fields.hpp"
+#include "hop_impl.hpp"
+
+namespace qphix_amd {
+
+constexpr int kBlasThreads = 256;
+
+__host__ __device__ constexpr int nred_of(int op) {
+  return (op == B_NORM2 || op == B_AXPYNORM2 || op == B_XMYNORM2 || op == B_XMY2NORM2 || op == B_RMAMMP ||
+          op == B_RICH || op == B_RXUPD)
+             ? 1
+             : (op == B_CDOT) ? 2 : (op == B_CDOTNORM) ? 3 : 0;
+}
+int blas_nred(int op) { return nred_of(op); }
+
+template <typename T> __device__ inline double sq(const T *v) {
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < kSpinorReals; k++) s += (double)v[k] * (double)v[k];
+  return s;
+}
+
+template <int P, int OP>
+__device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *red) {
+  typedef typename PT<P>::calc T;
+  constexpr int N = kSpinorReals;
+  T x[N], y[N];
+  const T a0 = (T)a.a[0], a1 = (T)a.a[1];
+  if constexpr (OP == B_COPY) {
+    load_spinor<P>(a.x, vs, s, x);
+    store_spinor<P>(a.o0, vs, s, x);
+  } else if constexpr (OP == B_ZERO) {
+#pragma unroll
+    for (int k = 0; k < N; k++) x[k] = 0;
+    store_spinor<P>(a.o0, vs, s, x);
+  } else if constexpr (OP == B_AXY) {  // y = a x
+    load_spinor<P>(a.x, vs, s, x);
+#pragma unroll
+    for (int k = 0; k < N; k++) y[k] = a0 * x[k];
+    store_spinor<P>(a.o0, vs, s, y);
+  } else if constexpr (OP == B_AX) {  // x = a x
+    load_spinor<P>(a.o0, vs, s, x);
+#pragma unroll
+    for (int k = 0; k < N; k++) x[k] = a0 * x[k];
+    store_spinor<P>(a.o0, vs, s, x);
+  } else if constexpr (OP == B_YPEQX || OP == B_YMEQX || OP == B_AYPX || OP == B_AXPY || OP == B_AXPBY ||
+                       OP == B_AXPYNORM2 || OP == B_XMY2NORM2 || OP == B_BICG_XMY) {
+    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P>(a.o0, vs, s, y);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      if constexpr (OP == B_YPEQX) y[k] = y[k] + x[k];
+      if constexpr (OP == B_YMEQX || OP == B_XMY2NORM2) y[k] = y[k] - x[k];
+      if constexpr (OP == B_AYPX) y[k] = a0 * y[k] + x[k];
+      if constexpr (OP == B_AXPY || OP == B_AXPYNORM2) y[k] = a0 * x[k] + y[k];
+      if constexpr (OP == B_AXPBY) y[k] = a0 * x[k] + a1 * y[k];
+      if constexpr (OP == B_BICG_XMY) y[k] = x[k] - y[k];
+    }
+    if constexpr (OP == B_AXPYNORM2 || OP == B_XMY2NORM2) red[0] += sq(y);
+    store_spinor<P>(a.o0, vs, s, y);
+  } else if constexpr (OP == B_NORM2) {
+    load_spinor<P>(a.x, vs, s, x);
+    red[0] += sq(x);
+  } else if constexpr (OP == B_XMYNORM2) {  // o0 = x - y
+    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P>(a.y, vs, s, y);
+#pragma unroll
+    for (int k = 0; k < N; k++) x[k] = x[k] - y[k];
+    red[0] += sq(x);
+    store_spinor<P>(a.o0, vs, s, x);
+  } else if constexpr (OP == B_RMAMMP) {  // r -= a mmp ; x += a p
+    T r[N], xx[N];
+    load_spinor<P>(a.x, vs, s, x);   // mmp
+    load_spinor<P>(a.o0, vs, s, r);  // r
+#pragma unroll
+    for (int k = 0; k < N; k++) r[k] = r[k] - a0 * x[k];
+    red[0] += sq(r);
+    store_spinor<P>(a.o0, vs, s, r);
+    load_spinor<P>(a.y, vs, s, y);    // p
+    load_spinor<P>(a.o1, vs, s, xx);  // x
+#pragma unroll
+    for (int k = 0; k < N; k++) xx[k] = xx[k] + a0 * y[k];
+    store_spinor<P>(a.o1, vs, s, xx);
+  } else if constexpr (OP == B_RICH) {  // x += dx ; r -= dr
+    T t[N];
+    load_spinor<P>(a.x, vs, s, x);   // dx
+    load_spinor<P>(a.o0, vs, s, t);  // x
+#pragma unroll
+    for (int k = 0; k < N; k++) t[k] = t[k] + x[k];
+    store_spinor<P>(a.o0, vs, s, t);
+    load_spinor<P>(a.y, vs, s, y);   // dr
+    load_spinor<P>(a.o1, vs, s, t);  // r
+#pragma unroll
+    for (int k = 0; k < N; k++) t[k] = t[k] - y[k];
+    red[0] += sq(t);
+    store_spinor<P>(a.o1, vs, s, t);
+  } else if constexpr (OP == B_CDOT || OP == B_CDOTNORM) {  // <x, y> = sum conj(x) y
+    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P>(a.y, vs, s, y);
+    double re = 0, im = 0;
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      re += (double)x[k] * (double)y[k] + (double)x[k + 1] * (double)y[k + 1];
+      im += (double)x[k] * (double)y[k + 1] - (double)x[k + 1] * (double)y[k];
+    }
+    red[0] += re;
+    red[1] += im;
+    if constexpr (OP == B_CDOTNORM) red[2] += sq(x);
+  } else if constexpr (OP == B_PUPD) {  // p = r + beta (p - omega v); x=r, y=v, o0=p, a = beta, omega
+    T v[N];
+    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P>(a.y, vs, s, v);
+    load_spinor<P>(a.o0, vs, s, y);
+    const T br = a0, bi = a1, wr = (T)a.a[2], wi = (T)a.a[3];
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const T tr = y[k] - (wr * v[k] - wi * v[k + 1]);
+      const T ti = y[k + 1] - (wr * v[k + 1] + wi * v[k]);
+      y[k] = x[k] + (br * tr - bi * ti);
+      y[k + 1] = x[k + 1] + (br * ti + bi * tr);
+    }
+    store_spinor<P>(a.o0, vs, s, y);
+  } else if constexpr (OP == B_SUPD || OP == B_CAXPY) {  // s -= alpha v ; y += alpha x
+    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P>(a.o0, vs, s, y);
+    const T sg = (OP == B_SUPD) ? (T)-1 : (T)1;
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const T pr = a0 * x[k] - a1 * x[k + 1], pi = a0 * x[k + 1] + a1 * x[k];
+      y[k] = y[k] + sg * pr;
+      y[k + 1] = y[k + 1] + sg * pi;
+    }
+    store_spinor<P>(a.o0, vs, s, y);
+  } else if constexpr (OP == B_RXUPD) {  // x += omega r + alpha p ; r -= omega t  (x=t, y=p, o0=x, o1=r)
+    T r[N], xx[N];
+    const T wr = a0, wi = a1, ar = (T)a.a[2], ai = (T)a.a[3];
+    load_spinor<P>(a.o1, vs, s, r);
+    load_spinor<P>(a.y, vs, s, y);
+    load_spinor<P>(a.o0, vs, s, xx);
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      xx[k] = xx[k] + (wr * r[k] - wi * r[k + 1]) + (ar * y[k] - ai * y[k + 1]);
+      xx[k + 1] = xx[k + 1] + (wr * r[k + 1] + wi * r[k]) + (ar * y[k + 1] + ai * y[k]);
+    }
+    store_spinor<P>(a.o0, vs, s, xx);
+    load_spinor<P>(a.x, vs, s, x);
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const T tr = wr * x[k] - wi * x[k + 1], ti = wr * x[k + 1] + wi * x[k];
+      r[k] = r[k] - tr;
+      r[k + 1] = r[k + 1] - ti;
+    }
+    red[0] += sq(r);
+    store_spinor<P>(a.o1, vs, s, r);
+  } else if constexpr (OP == B_MRUPD) {  // x += a r ; r -= a Mr   (x = Mr, o0 = x, o1 = r)
+    T r[N], xx[N];
+    load_spinor<P>(a.o1, vs, s, r);
+    load_spinor<P>(a.o0, vs, s, xx);
+    load_spinor<P>(a.x, vs, s, x);
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      xx[k] = xx[k] + (a0 * r[k] - a1 * r[k + 1]);
+      xx[k + 1] = xx[k + 1] + (a0 * r[k + 1] + a1 * r[k]);
+      r[k] = r[k] - (a0 * x[k] - a1 * x[k + 1]);
+      r[k + 1] = r[k + 1] - (a0 * x[k + 1] + a1 * x[k]);
+    }
+    store_spinor<P>(a.o0, vs, s, xx);
+    store_spinor<P>(a.o1, vs, s, r);
+  } else if constexpr (OP == B_TWM) {  // y = (a0 + i a1 g5) x
+    load_spinor<P>(a.x, vs, s, x);
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const T g = (k < 12) ? a1 : -a1;
+      y[k] = a0 * x[k] - g * x[k + 1];
+      y[k + 1] = a0 * x[k + 1] + g * x[k];
+    }
+    store_spinor<P>(a.o0, vs, s, y);
+  } else if constexpr (OP == B_TWM2) {  // (a0 + i a1 g5 tau3 + eps tau1): x=up, y=dn, o0, o1
+    T u[N], d[N], ou[N], od[N];
+    const T e = (T)a.a[2];
+    load_spinor<P>(a.x, vs, s, u);
+    load_spinor<P>(a.y, vs, s, d);
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const T g = (k < 12) ? a1 : -a1;
+      ou[k] = a0 * u[k] - g * u[k + 1] + e * d[k];
+      ou[k + 1] = a0 * u[k + 1] + g * u[k] + e * d[k + 1];
+      od[k] = a0 * d[k] + g * d[k + 1] + e * u[k];
+      od[k + 1] = a0 * d[k + 1] - g * d[k] + e * u[k + 1];
+    }
+    store_spinor<P>(a.o0, vs, s, ou);
+    store_spinor<P>(a.o1, vs, s, od);
+  } else if constexpr (OP == B_CLOVMULT || OP == B_FULLMULT) {  // o0 = A x  (y = clover)
+    cx<T> in[4][3], out[4][3];
+    load_spinor<P>(a.x, vs, s, x);
+    to_spinor(x, in);
+    if constexpr (OP == B_CLOVMULT) {
+      clover_block_apply<P, 0>(a.y, vs, s, in, out);
+      clover_block_apply<P, 1>(a.y, vs, s, in, out);
+    } else {
+      full_block_apply<P, 0>(a.y, vs, s, in, out);
+      full_block_apply<P, 1>(a.y, vs, s, in, out);
+    }
+    from_spinor(out, y);
+    store_spinor<P>(a.o0, vs, s, y);
+  } else if constexpr (OP == B_TWOFLAV_INVCLOV) {
+    // o0 = Fu x + C y ; o1 = Fd y + C x   (x = psi_up, y = psi_dn, z = Fu, w = Fd, o2 = C)
+    cx<T> pu[4][3], pd[4][3], r[4][3], c[4][3];
+    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P>(a.y, vs, s, y);
+    to_spinor(x, pu);
+    to_spinor(y, pd);
+    full_block_apply<P, 0>(a.z, vs, s, pu, r);
+    full_block_apply<P, 1>(a.z, vs, s, pu, r);
+    clover_block_apply<P, 0>(a.o2, vs, s, pd, c);
+    clover_block_apply<P, 1>(a.o2, vs, s, pd, c);
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) r[sp][cc] = r[sp][cc] + c[sp][cc];
+    T o[N];
+    from_spinor(r, o);
+    store_spinor<P>(a.o0, vs, s, o);
+    full_block_apply<P, 0>(a.w, vs, s, pd, r);
+    full_block_apply<P, 1>(a.w, vs, s, pd, r);
+    clover_block_apply<P, 0>(a.o2, vs, s, pu, c);
+    clover_block_apply<P, 1>(a.o2, vs, s, pu, c);
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) r[sp][cc] = r[sp][cc] + c[sp][cc];
+    from_spinor(r, o);
+    store_spinor<P>(a.o1, vs, s, o);
+  }
+}
+
+template <int NR> __device__ inline void block_reduce_store(double *red, double *partials) {
+  __shared__ double lds[kBlasThreads / 64][NR > 0 ? NR : 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double v = red[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) lds[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double v = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += lds[w][threadIdx.x];
+    partials[blockIdx.x * NR + threadIdx.x] = v;
+  }
+}
+
+template <int P, int OP>
+__global__ void __launch_bounds__(kBlasThreads) k_blas(const BlasArgs a, size_t vs, long vh, double *partials) {
+  constexpr int NR = nred_of(OP);
+  double red[NR > 0 ? NR : 1] = {0};
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < vh; s += stride) blas_site<P, OP>(a, vs, s, red);
+  if constexpr (NR > 0) block_reduce_store<NR>(red, partials);
+}
+
+template <int NR>
+__global__ void __launch_bounds__(kBlasThreads) k_finalize(const double *partials, int nblocks, double *out) {
+  double red[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) red[k] = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NR; k++) red[k] += partials[b * NR + k];
+  __shared__ double lds[kBlasThreads / 64][NR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double v = red[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) lds[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double v = 0;
+    for (int w = 0; w < kBlasThreads / 64; w++) v += lds[w][threadIdx.x];
+    out[threadIdx.x] = v;
+  }
+}
+
+template <int P, int OP>
+static void launch_one(const GeomDesc &g, const BlasArgs &a, double *partials, int nb, hipStream_t st) {
+  k_blas<P, OP><<<nb, kBlasThreads, 0, st>>>(a, g.vs, g.vh, partials);
+}
+
+template <int P>
+static hipError_t launch_prec(const GeomDesc &g, int op, const BlasArgs &a, double *partials, int nb, hipStream_t st) {
+#define OPCASE(o) \
+  case o: launch_one<P, o>(g, a, partials, nb, st); break;
+  switch (op) {
+    OPCASE(B_COPY) OPCASE(B_ZERO) OPCASE(B_AXY) OPCASE(B_AX) OPCASE(B_YPEQX) OPCASE(B_YMEQX)
+    OPCASE(B_AYPX) OPCASE(B_AXPY) OPCASE(B_AXPBY) OPCASE(B_NORM2) OPCASE(B_AXPYNORM2) OPCASE(B_XMYNORM2)
+    OPCASE(B_XMY2NORM2) OPCASE(B_RMAMMP) OPCASE(B_RICH) OPCASE(B_BICG_XMY) OPCASE(B_CDOT) OPCASE(B_CDOTNORM)
+    OPCASE(B_PUPD) OPCASE(B_SUPD) OPCASE(B_CAXPY) OPCASE(B_RXUPD) OPCASE(B_MRUPD) OPCASE(B_TWM)
+    OPCASE(B_TWM2) OPCASE(B_CLOVMULT) OPCASE(B_FULLMULT) OPCASE(B_TWOFLAV_INVCLOV)
+    default: return hipErrorInvalidValue;
+  }
+#undef OPCASE
+  return hipGetLastError();
+}
+
+hipError_t blas_launch(const GeomDesc &g, int op, const BlasArgs &a, double *partials, int *nblocks, hipStream_t st) {
+  long nb = (g.vh + kBlasThreads - 1) / kBlasThreads;
+  if (nb > kMaxBlasBlocks) nb = kMaxBlasBlocks;
+  if (nb < 1) nb = 1;
+  *nblocks = (int)nb;
+  switch (g.prec) {
+    case P64: return launch_prec<P64>(g, op, a, partials, (int)nb, st);
+    case P32: return launch_prec<P32>(g, op, a, partials, (int)nb, st);
+    case P16: return launch_prec<P16>(g, op, a, partials, (int)nb, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t blas_finalize(const double *partials, int nblocks, int nred, double *out, hipStream_t st) {
+  switch (nred) {
+    case 1: k_finalize<1><<<1, kBlasThreads, 0, st>>>(partials, nblocks, out); break;
+    case 2: k_finalize<2><<<1, kBlasThreads, 0, st>>>(partials, nblocks, out); break;
+    case 3: k_finalize<3><<<1, kBlasThreads, 0, st>>>(partials, nblocks, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace qphix_amd

@@ -1,0 +1,234 @@
+// Shared device/host definitions for the gfx950 Dslash library.
+//
+// Field layout in HBM (site-major structure of arrays):
+//   A field is an array of W "words" of 16 bytes per site. Word w of
+//   checkerboard site s lives at byte offset (w * vs + s) * 16, where vs is the
+//   per-plane stride (Vh rounded up to a multiple of 256 sites). Consecutive
+//   lanes of a wavefront read consecutive sites, so each word load is one
+//   fully coalesced 1 KiB transaction per wavefront.
+//   A word holds 2 doubles (F64), 4 floats (F32) or 8 halves (F16). F16 fields
+//   carry one float scale per site after the last word plane
+//   (byte offset W * vs * 16 + s * 4); the stored halves are value / scale.
+//
+// Per-site real counts (same real order as the canonical host layout):
+//   spinor        24   [spin 4][colour 3][re,im]
+//   gauge         144  (uncompressed) or 96 (two-row): 8 links, link
+//                 l = 2*mu + (0: U_mu(x - mu), 1: U_mu(x)), each [row][col][re,im]
+//                 row-major U (rows 0,1 only when two-row compressed)
+//   clover        72   per block b: diag[6], then offd[15][re,im] (lower
+//                 triangle, offd[i(i-1)/2 + j] = M(i,j), i > j)
+//   full clover   144  per block b: M[6][6][re,im]
+// The checkerboard site index is s = ((t*Nz + z)*Ny + y)*Nxh + x/2 with
+// x = 2*(x/2) + ((y + z + t + cb) & 1), which is the canonical order of the
+// QDP++ packers (include/qphix/qdp_packer_parscalar.h:75).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qphix_amd {
+
+enum Prec { P64 = 0, P32 = 1, P16 = 2 };
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+template <int P> struct PT;
+template <> struct PT<P64> {
+  typedef double calc;
+  static constexpr int per_word = 2;
+  static constexpr bool scaled = false;
+};
+template <> struct PT<P32> {
+  typedef float calc;
+  static constexpr int per_word = 4;
+  static constexpr bool scaled = false;
+};
+template <> struct PT<P16> {
+  typedef float calc;
+  static constexpr int per_word = 8;
+  static constexpr bool scaled = true;
+};
+
+constexpr int kSpinorReals = 24;
+constexpr int kCloverReals = 72;
+constexpr int kFullCloverReals = 144;
+__host__ __device__ constexpr int gauge_reals(bool c12) { return c12 ? 96 : 144; }
+
+template <int P> __host__ __device__ constexpr int words_of(int nreal) { return nreal / PT<P>::per_word; }
+
+// bytes of one field with nreal reals per site and plane stride vs
+template <int P> __host__ __device__ inline size_t field_bytes(int nreal, size_t vs) {
+  return (size_t)words_of<P>(nreal) * vs * 16 + (PT<P>::scaled ? vs * 4 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// word loads / stores
+// ---------------------------------------------------------------------------
+template <int P> struct Word;
+
+template <> struct Word<P64> {
+  __device__ static inline void load(const void *base, size_t vs, long s, int w, double *o) {
+    const double2 v = reinterpret_cast<const double2 *>(base)[(size_t)w * vs + s];
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+  __device__ static inline void store(void *base, size_t vs, long s, int w, const double *i) {
+    double2 v;
+    v.x = i[0];
+    v.y = i[1];
+    reinterpret_cast<double2 *>(base)[(size_t)w * vs + s] = v;
+  }
+};
+
+template <> struct Word<P32> {
+  __device__ static inline void load(const void *base, size_t vs, long s, int w, float *o) {
+    const float4 v = reinterpret_cast<const float4 *>(base)[(size_t)w * vs + s];
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  }
+  __device__ static inline void store(void *base, size_t vs, long s, int w, const float *i) {
+    float4 v;
+    v.x = i[0];
+    v.y = i[1];
+    v.z = i[2];
+    v.w = i[3];
+    reinterpret_cast<float4 *>(base)[(size_t)w * vs + s] = v;
+  }
+};
+
+template <> struct Word<P16> {
+  // loads raw (unscaled) values; callers multiply by the site scale
+  __device__ static inline void load(const void *base, size_t vs, long s, int w, float *o) {
+    const half8_t v = reinterpret_cast<const half8_t *>(base)[(size_t)w * vs + s];
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = (float)v[k];
+  }
+  __device__ static inline void store(void *base, size_t vs, long s, int w, const float *i) {
+    half8_t v;
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = (_Float16)i[k];
+    reinterpret_cast<half8_t *>(base)[(size_t)w * vs + s] = v;
+  }
+};
+
+template <int P> __device__ inline float site_scale(const void *base, size_t vs, long s, int nwords) {
+  const float *sc = reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + (size_t)nwords * vs * 16);
+  return sc[s];
+}
+
+// Load reals [W0*per_word, (W0+NW)*per_word) of a field with NWT words per site.
+template <int P, int NWT, int W0, int NW>
+__device__ inline void load_range(const void *base, size_t vs, long s, typename PT<P>::calc *o) {
+  typedef typename PT<P>::calc T;
+#pragma unroll
+  for (int w = 0; w < NW; w++) Word<P>::load(base, vs, s, W0 + w, o + w * PT<P>::per_word);
+  if constexpr (PT<P>::scaled) {
+    T sc = site_scale<P>(base, vs, s, NWT);
+#pragma unroll
+    for (int k = 0; k < NW * PT<P>::per_word; k++) o[k] *= sc;
+  }
+}
+
+// Store all NWT words of a site (computes the F16 per-site scale).
+template <int P, int NWT>
+__device__ inline void store_all(void *base, size_t vs, long s, const typename PT<P>::calc *v) {
+  constexpr int N = NWT * PT<P>::per_word;
+  if constexpr (PT<P>::scaled) {
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; k++) m = fmaxf(m, fabsf(v[k]));
+    float inv = m > 0.f ? 1.0f / m : 0.f;
+    float tmp[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) tmp[k] = v[k] * inv;
+#pragma unroll
+    for (int w = 0; w < NWT; w++) Word<P>::store(base, vs, s, w, tmp + w * PT<P>::per_word);
+    float *sc = reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (size_t)NWT * vs * 16);
+    sc[s] = m;
+  } else {
+#pragma unroll
+    for (int w = 0; w < NWT; w++) Word<P>::store(base, vs, s, w, v + w * PT<P>::per_word);
+  }
+}
+
+template <int P>
+__device__ inline void load_spinor(const void *base, size_t vs, long s, typename PT<P>::calc *o) {
+  load_range<P, words_of<P>(kSpinorReals), 0, words_of<P>(kSpinorReals)>(base, vs, s, o);
+}
+template <int P>
+__device__ inline void store_spinor(void *base, size_t vs, long s, const typename PT<P>::calc *v) {
+  store_all<P, words_of<P>(kSpinorReals)>(base, vs, s, v);
+}
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+template <typename T> struct cx {
+  T r, i;
+};
+template <typename T> __device__ inline cx<T> mk(T r, T i) { cx<T> c; c.r = r; c.i = i; return c; }
+template <typename T> __device__ inline cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.r + b.r, a.i + b.i); }
+template <typename T> __device__ inline cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.r - b.r, a.i - b.i); }
+template <typename T> __device__ inline cx<T> operator*(cx<T> a, cx<T> b) {
+  return mk(a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r);
+}
+template <typename T> __device__ inline cx<T> cmulc(cx<T> a, cx<T> b) {  // conj(a) * b
+  return mk(a.r * b.r + a.i * b.i, a.r * b.i - a.i * b.r);
+}
+template <typename T> __device__ inline cx<T> scale(T s, cx<T> a) { return mk(s * a.r, s * a.i); }
+template <typename T> __device__ inline cx<T> conj(cx<T> a) { return mk(a.r, -a.i); }
+
+// multiply by a compile-time phase in {1, -1, i, -i} given as (PR, PI)
+template <int PR, int PI, typename T> __device__ inline cx<T> mulph(cx<T> a) {
+  if constexpr (PI == 0) {
+    if constexpr (PR == 1) return a;
+    else return mk(-a.r, -a.i);
+  } else {
+    if constexpr (PI == 1) return mk(-a.i, a.r);
+    else return mk(a.i, -a.r);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DeGrand-Rossi gamma matrices as (column, phase) per row
+// (the basis of codegen/dslash_common.cc:848-990; gamma_5 = diag(1,1,-1,-1))
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int gcol(int mu, int row) {
+  return (mu < 2) ? (3 - row) : ((row + 2) & 3);
+}
+// phase real / imag parts
+__host__ __device__ constexpr int gpr(int mu, int row) {
+  return mu == 0 ? 0 : mu == 1 ? ((row == 0 || row == 3) ? -1 : 1) : mu == 2 ? 0 : 1;
+}
+__host__ __device__ constexpr int gpi(int mu, int row) {
+  return mu == 0 ? (row < 2 ? 1 : -1) : mu == 1 ? 0 : mu == 2 ? ((row == 0 || row == 3) ? 1 : -1) : 0;
+}
+
+// h = rows 0,1 of (1 + S*gamma_mu) psi, psi as cx[4][3]
+template <int MU, int S, typename T>
+__device__ inline void project(const cx<T> (&psi)[4][3], cx<T> (&h)[2][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    h[0][c] = psi[0][c] + mulph<S * gpr(MU, 0), S * gpi(MU, 0)>(psi[gcol(MU, 0)][c]);
+    h[1][c] = psi[1][c] + mulph<S * gpr(MU, 1), S * gpi(MU, 1)>(psi[gcol(MU, 1)][c]);
+  }
+}
+
+// acc += coef * (1 + S*gamma_mu)-reconstruction of the colour-rotated h
+template <int MU, int S, typename T>
+__device__ inline void recon_acc(cx<T> (&acc)[4][3], const cx<T> (&h)[2][3], T coef) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    cx<T> a = scale(coef, h[0][c]);
+    cx<T> b = scale(coef, h[1][c]);
+    acc[0][c] = acc[0][c] + a;
+    acc[1][c] = acc[1][c] + b;
+    // row k (k = 2,3) = S * phase(k) * h[col(k)]
+    acc[2][c] = acc[2][c] + mulph<S * gpr(MU, 2), S * gpi(MU, 2)>(gcol(MU, 2) == 0 ? a : b);
+    acc[3][c] = acc[3][c] + mulph<S * gpr(MU, 3), S * gpi(MU, 3)>(gcol(MU, 3) == 0 ? a : b);
+  }
+}
+
+}  // namespace qphix_amd

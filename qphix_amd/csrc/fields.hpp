@@ -1,0 +1,54 @@
+// Host-side declarations shared by the library translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace qphix_amd {
+
+// lattice + precision descriptor of a geometry
+struct GeomDesc {
+  int L[4];
+  int prec;   // Prec
+  int c12;    // two-row gauge compression
+  long vh;    // sites per checkerboard
+  size_t vs;  // plane stride in sites
+};
+
+hipError_t fld_pack_spinor(const GeomDesc &g, void *dst, const double *dsrc, hipStream_t st);
+hipError_t fld_unpack_spinor(const GeomDesc &g, double *ddst, const void *src, hipStream_t st);
+hipError_t fld_gen_spinor(const GeomDesc &g, void *dst, uint64_t seed, int cb, hipStream_t st);
+hipError_t fld_convert_spinor(const GeomDesc &go, void *out, double scale, const GeomDesc &gi, const void *in,
+                              hipStream_t st);
+hipError_t fld_pack_gauge_cb(const GeomDesc &g, void *dst, int cb, const double *dsrc, hipStream_t st);
+hipError_t fld_gen_gauge_canonical(const GeomDesc &g, double *du, uint64_t seed, double eps, hipStream_t st);
+hipError_t fld_convert_gauge(const GeomDesc &go, void *out, const GeomDesc &gi, const void *in, hipStream_t st);
+hipError_t fld_pack_clover(const GeomDesc &g, void *dst, const double *dsrc, hipStream_t st);
+hipError_t fld_unpack_clover(const GeomDesc &g, double *ddst, const void *src, hipStream_t st);
+hipError_t fld_pack_full_clover(const GeomDesc &g, void *dst, const double *dsrc, hipStream_t st);
+hipError_t fld_convert_clover(const GeomDesc &go, void *out, const GeomDesc &gi, const void *in, int full,
+                              hipStream_t st);
+hipError_t fld_clover_build(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct, double dm,
+                            int cb, int what, double mu, double eps, void *out, hipStream_t st);
+
+// BLAS operations (one spinor per operand; see blas.hip)
+enum BlasOp {
+  B_COPY = 0, B_ZERO, B_AXY, B_AX, B_YPEQX, B_YMEQX, B_AYPX, B_AXPY, B_AXPBY,
+  B_NORM2, B_AXPYNORM2, B_XMYNORM2, B_XMY2NORM2, B_RMAMMP, B_RICH, B_BICG_XMY,
+  B_CDOT, B_CDOTNORM, B_PUPD, B_SUPD, B_CAXPY, B_RXUPD, B_MRUPD, B_TWM, B_TWM2,
+  B_CLOVMULT, B_FULLMULT, B_TWOFLAV_INVCLOV, NUM_BLAS_OPS
+};
+struct BlasArgs {
+  const void *x, *y, *z, *w;  // read-only operands
+  void *o0, *o1, *o2, *o3;    // written (and possibly read) operands
+  double a[6];                // scalars
+};
+int blas_nred(int op);
+// launches op over the geometry; writes nred partial sums per block to
+// partials and returns the number of blocks used
+hipError_t blas_launch(const GeomDesc &g, int op, const BlasArgs &a, double *partials, int *nblocks, hipStream_t st);
+hipError_t blas_finalize(const double *partials, int nblocks, int nred, double *out, hipStream_t st);
+constexpr int kMaxBlasBlocks = 2048;
+
+}  // namespace qphix_amd

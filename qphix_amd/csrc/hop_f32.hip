@@ -1,0 +1,6 @@
+// Hop kernel instantiations for P32 storage.
+#include "hop_launch.hpp"
+
+namespace qphix_amd {
+template hipError_t launch_hop_prec<P32>(int, bool, int, const HopArgs &, int, hipStream_t);
+}

@@ -1,0 +1,344 @@
+// Even-odd hopping-term kernels for gfx950.
+//
+// One work-item per target-checkerboard site. For every dimension mu the
+// kernel gathers the two neighbour spinors of the source checkerboard, projects
+// them to half spinors with (1 -/+ isign gamma_mu), multiplies by U_mu(x) or
+// U_mu(x-mu)^+, and accumulates the reconstructed spinor with the hop
+// coefficient. The fused epilogues cover the whole kernel family of the
+// reference (codegen dslash_common.cc:1967-2183):
+//   KD_WILSON   res = hop                                      (dslash_body.h:338)
+//   KA_WILSON   res = alpha chi - beta hop                     (dslash_body.h:361)
+//   KD_CLOVER   res = A^-1 hop                                  (clover_dslash_body.h:339)
+//   KA_CLOVER   res = A chi + i isign rho g5 chi - beta hop     (clover_dslash_body.h:351)
+//   KD_TM       res = muinv (1 - i isign mu' g5) hop            (tm_dslash_body.h:344)
+//   KA_TM       res = alpha (1 + i isign mu' g5) chi - beta hop (tm_dslash_body.h:356)
+//   KD_FULL     res = F hop, F a full 6x6 block per chirality   (tm_clov_dslash_body.h:339)
+// The -beta factor is folded into the hop coefficients by the host.
+#pragma once
+
+#include "common.hpp"
+
+namespace qphix_amd {
+
+enum HopKind { KD_WILSON = 0, KA_WILSON, KD_CLOVER, KA_CLOVER, KD_TM, KA_TM, KD_FULL, NUM_HOP_KINDS };
+
+struct HopArgs {
+  const void *psi;
+  void *out;
+  const void *u;     // gauge array of the target checkerboard
+  const void *clov;  // clover / full clover of the target checkerboard
+  const void *chi;
+  size_t vs;
+  int vh, nxh, ny, nz, nt, cb;
+  double cs;         // spatial hop coefficient
+  double ct;         // temporal coefficient away from the boundary
+  double ctf, ctb;   // temporal coefficient of the forward hop at t = Nt-1 / backward hop at t = 0
+  double tbc_r[4], tbc_i[4];
+  int tbc_mask;
+  double alpha, mu, muinv, rho;
+};
+
+// block -> lattice-tile remap so that each XCD (8 per device, round-robin
+// dispatch) walks one contiguous range of sites and keeps its L2 warm
+__device__ inline long xcd_site(int block_threads) {
+  const int nb = gridDim.x, bid = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, xcd = bid & 7, loc = bid >> 3;
+  const int lb = (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+  return (long)lb * block_threads + threadIdx.x;
+}
+
+template <typename T> __device__ inline void to_spinor(const T *v, cx<T> (&p)[4][3]) {
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) p[s][c] = mk(v[(3 * s + c) * 2], v[(3 * s + c) * 2 + 1]);
+}
+template <typename T> __device__ inline void from_spinor(const cx<T> (&p)[4][3], T *v) {
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      v[(3 * s + c) * 2] = p[s][c].r;
+      v[(3 * s + c) * 2 + 1] = p[s][c].i;
+    }
+}
+
+// 3x3 link from 18 reals, or from 12 (rows 0,1) with row 2 = conj(row0 x row1)
+template <bool C12, typename T> __device__ inline void make_link(const T *v, cx<T> (&U)[3][3]) {
+#pragma unroll
+  for (int r = 0; r < (C12 ? 2 : 3); r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) U[r][c] = mk(v[(3 * r + c) * 2], v[(3 * r + c) * 2 + 1]);
+  if constexpr (C12) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int a = (c + 1) % 3, b = (c + 2) % 3;
+      U[2][c] = conj(U[0][a] * U[1][b] - U[0][b] * U[1][a]);
+    }
+  }
+}
+
+template <int P, bool C12> struct GaugeLayout {
+  static constexpr int link_reals = C12 ? 12 : 18;
+  static constexpr int nw = words_of<P>(gauge_reals(C12));
+  static constexpr bool per_link = (link_reals % PT<P>::per_word) == 0;
+  static constexpr int link_words = link_reals / PT<P>::per_word;  // valid if per_link
+  static constexpr int pair_words = 2 * link_reals / PT<P>::per_word;
+};
+
+template <int MU, int S, bool FWD, bool C12, typename T>
+__device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3], T coef, bool tbc,
+                                 T tr, T ti, cx<T> (&acc)[4][3]) {
+  cx<T> h[2][3], uh[2][3];
+  project<MU, S>(psi, h);
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      cx<T> sum;
+      if constexpr (FWD) {
+        sum = U[i][0] * h[r][0];
+        sum = sum + U[i][1] * h[r][1];
+        sum = sum + U[i][2] * h[r][2];
+      } else {
+        sum = cmulc(U[0][i], h[r][0]);
+        sum = sum + cmulc(U[1][i], h[r][1]);
+        sum = sum + cmulc(U[2][i], h[r][2]);
+      }
+      uh[r][i] = sum;
+    }
+  if (tbc) {
+    const cx<T> ph = mk(tr, FWD ? ti : -ti);
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) uh[r][i] = ph * uh[r][i];
+  }
+  recon_acc<MU, S>(acc, uh, coef);
+}
+
+template <int P, bool C12, int ISIGN, int MU>
+__device__ inline void hop_dim(const HopArgs &a, long s, long sb, long sf, typename PT<P>::calc cb_coef,
+                               typename PT<P>::calc cf_coef, cx<typename PT<P>::calc> (&acc)[4][3]) {
+  typedef typename PT<P>::calc T;
+  typedef GaugeLayout<P, C12> GL;
+  const bool tbc = (a.tbc_mask >> MU) & 1;
+  const T tr = (T)a.tbc_r[MU], ti = (T)a.tbc_i[MU];
+  T sp[kSpinorReals];
+  cx<T> psi[4][3], U[3][3];
+  if constexpr (GL::per_link) {
+    T lv[GL::link_reals];
+    // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
+    load_spinor<P>(a.psi, a.vs, sb, sp);
+    load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words>(a.u, a.vs, s, lv);
+    to_spinor(sp, psi);
+    make_link<C12>(lv, U);
+    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
+    // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
+    load_spinor<P>(a.psi, a.vs, sf, sp);
+    load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words>(a.u, a.vs, s, lv);
+    to_spinor(sp, psi);
+    make_link<C12>(lv, U);
+    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tbc, tr, ti, acc);
+  } else {
+    T lv[2 * GL::link_reals];
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words>(a.u, a.vs, s, lv);
+    load_spinor<P>(a.psi, a.vs, sb, sp);
+    to_spinor(sp, psi);
+    make_link<C12>(lv, U);
+    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
+    load_spinor<P>(a.psi, a.vs, sf, sp);
+    to_spinor(sp, psi);
+    make_link<C12>(lv + GL::link_reals, U);
+    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tbc, tr, ti, acc);
+  }
+}
+
+// --- clover blocks --------------------------------------------------------
+// packed Hermitian block b (36 reals) of a 72-real clover site
+template <int P, int B> __device__ inline void load_clover_block(const void *c, size_t vs, long s,
+                                                                 typename PT<P>::calc *o) {
+  constexpr int nw = words_of<P>(kCloverReals);
+  if constexpr ((36 % PT<P>::per_word) == 0) {
+    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word>(c, vs, s, o);
+  } else {
+    typename PT<P>::calc all[kCloverReals];
+    load_range<P, nw, 0, nw>(c, vs, s, all);
+#pragma unroll
+    for (int k = 0; k < 36; k++) o[k] = all[36 * B + k];
+  }
+}
+
+// out = M in for a packed Hermitian 6x6 block m = {diag[6], offd[15][2]}
+template <typename T> __device__ inline void herm6(const T *m, const cx<T> (&in)[6], cx<T> (&out)[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    cx<T> s = scale(m[i], in[i]);
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      if (j < i) {
+        const int e = i * (i - 1) / 2 + j;
+        s = s + mk(m[6 + 2 * e], m[6 + 2 * e + 1]) * in[j];
+      } else if (j > i) {
+        const int e = j * (j - 1) / 2 + i;
+        s = s + cmulc(mk(m[6 + 2 * e], m[6 + 2 * e + 1]), in[j]);
+      }
+    }
+    out[i] = s;
+  }
+}
+
+template <typename T> __device__ inline void full6(const T *m, const cx<T> (&in)[6], cx<T> (&out)[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    cx<T> s = mk(m[12 * i], m[12 * i + 1]) * in[0];
+#pragma unroll
+    for (int j = 1; j < 6; j++) s = s + mk(m[12 * i + 2 * j], m[12 * i + 2 * j + 1]) * in[j];
+    out[i] = s;
+  }
+}
+
+template <int P, int B, typename T>
+__device__ inline void clover_block_apply(const void *clov, size_t vs, long s, const cx<T> (&in)[4][3],
+                                          cx<T> (&out)[4][3]) {
+  T m[36];
+  load_clover_block<P, B>(clov, vs, s, m);
+  cx<T> v[6], w[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) v[i] = in[2 * B + i / 3][i % 3];
+  herm6(m, v, w);
+#pragma unroll
+  for (int i = 0; i < 6; i++) out[2 * B + i / 3][i % 3] = w[i];
+}
+
+template <int P, int B, typename T>
+__device__ inline void full_block_apply(const void *fcl, size_t vs, long s, const cx<T> (&in)[4][3],
+                                        cx<T> (&out)[4][3]) {
+  constexpr int nw = words_of<P>(kFullCloverReals);
+  constexpr int bw = nw / 2;
+  T m[72];
+  load_range<P, nw, B * bw, bw>(fcl, vs, s, m);
+  cx<T> v[6], w[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) v[i] = in[2 * B + i / 3][i % 3];
+  full6(m, v, w);
+#pragma unroll
+  for (int i = 0; i < 6; i++) out[2 * B + i / 3][i % 3] = w[i];
+}
+
+// out += i * g * gamma_5 in  (g5 = +1 on spins 0,1 and -1 on spins 2,3)
+template <typename T> __device__ inline void add_ig5(T g, const cx<T> (&in)[4][3], cx<T> (&out)[4][3]) {
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++) {
+    const T gg = sp < 2 ? g : -g;
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[sp][c] = out[sp][c] + mk(-gg * in[sp][c].i, gg * in[sp][c].r);
+  }
+}
+
+template <int P, bool C12, int ISIGN, int KIND>
+__global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
+  typedef typename PT<P>::calc T;
+  const long s = xcd_site(blockDim.x);
+  if (s >= a.vh) return;
+
+  // coordinates of the target site
+  int r = (int)s;
+  const int xh = r % a.nxh;
+  r /= a.nxh;
+  const int y = r % a.ny;
+  r /= a.ny;
+  const int z = r % a.nz;
+  const int t = r / a.nz;
+  const int nx = 2 * a.nxh;
+  const int x = 2 * xh + ((y + z + t + a.cb) & 1);
+
+  const long row = s - xh;
+  const long xb = row + ((x == 0 ? nx - 1 : x - 1) >> 1);
+  const long xf = row + ((x == nx - 1 ? 0 : x + 1) >> 1);
+  const long sy = a.nxh, sz = (long)a.nxh * a.ny, st = sz * a.nz;
+  const long yb = y == 0 ? s + (a.ny - 1) * sy : s - sy;
+  const long yf = y == a.ny - 1 ? s - (a.ny - 1) * sy : s + sy;
+  const long zb = z == 0 ? s + (a.nz - 1) * sz : s - sz;
+  const long zf = z == a.nz - 1 ? s - (a.nz - 1) * sz : s + sz;
+  const long tb = t == 0 ? s + (a.nt - 1) * st : s - st;
+  const long tf = t == a.nt - 1 ? s - (a.nt - 1) * st : s + st;
+
+  const T cs = (T)a.cs;
+  const T ctb = (T)(t == 0 ? a.ctb : a.ct);
+  const T ctf = (T)(t == a.nt - 1 ? a.ctf : a.ct);
+
+  cx<T> acc[4][3];
+  if constexpr (KIND == KA_WILSON || KIND == KA_CLOVER || KIND == KA_TM) {
+    T cv[kSpinorReals];
+    cx<T> chi[4][3];
+    load_spinor<P>(a.chi, a.vs, s, cv);
+    to_spinor(cv, chi);
+    if constexpr (KIND == KA_WILSON) {
+      const T al = (T)a.alpha;
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[sp][c] = scale(al, chi[sp][c]);
+    } else if constexpr (KIND == KA_CLOVER) {
+      clover_block_apply<P, 0>(a.clov, a.vs, s, chi, acc);
+      clover_block_apply<P, 1>(a.clov, a.vs, s, chi, acc);
+      add_ig5((T)(ISIGN * a.rho), chi, acc);
+    } else {
+      const T al = (T)a.alpha;
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[sp][c] = chi[sp][c];
+      add_ig5((T)(ISIGN * a.mu), chi, acc);
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[sp][c] = scale(al, acc[sp][c]);
+    }
+  } else {
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[sp][c] = mk((T)0, (T)0);
+  }
+
+  hop_dim<P, C12, ISIGN, 0>(a, s, xb, xf, cs, cs, acc);
+  hop_dim<P, C12, ISIGN, 1>(a, s, yb, yf, cs, cs, acc);
+  hop_dim<P, C12, ISIGN, 2>(a, s, zb, zf, cs, cs, acc);
+  hop_dim<P, C12, ISIGN, 3>(a, s, tb, tf, ctb, ctf, acc);
+
+  T ov[kSpinorReals];
+  if constexpr (KIND == KD_CLOVER) {
+    cx<T> o[4][3];
+    clover_block_apply<P, 0>(a.clov, a.vs, s, acc, o);
+    clover_block_apply<P, 1>(a.clov, a.vs, s, acc, o);
+    from_spinor(o, ov);
+  } else if constexpr (KIND == KD_FULL) {
+    cx<T> o[4][3];
+    full_block_apply<P, 0>(a.clov, a.vs, s, acc, o);
+    full_block_apply<P, 1>(a.clov, a.vs, s, acc, o);
+    from_spinor(o, ov);
+  } else if constexpr (KIND == KD_TM) {
+    cx<T> o[4][3];
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) o[sp][c] = acc[sp][c];
+    add_ig5((T)(-ISIGN * a.mu), acc, o);
+    const T mi = (T)a.muinv;
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) o[sp][c] = scale(mi, o[sp][c]);
+    from_spinor(o, ov);
+  } else {
+    from_spinor(acc, ov);
+  }
+  store_spinor<P>(a.out, a.vs, s, ov);
+}
+
+template <int P>
+hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int block, hipStream_t st);
+
+}  // namespace qphix_amd

@@ -1,0 +1,34 @@
+// Host-side launch of the hop kernel family for one storage precision.
+#pragma once
+
+#include "hop_impl.hpp"
+
+namespace qphix_amd {
+
+template <int P, bool C12, int ISIGN>
+static hipError_t launch_hop3(int kind, const HopArgs &a, int block, hipStream_t st) {
+  const dim3 grid((unsigned)((a.vh + block - 1) / block));
+  switch (kind) {
+    case KD_WILSON: hop_kernel<P, C12, ISIGN, KD_WILSON><<<grid, block, 0, st>>>(a); break;
+    case KA_WILSON: hop_kernel<P, C12, ISIGN, KA_WILSON><<<grid, block, 0, st>>>(a); break;
+    case KD_CLOVER: hop_kernel<P, C12, ISIGN, KD_CLOVER><<<grid, block, 0, st>>>(a); break;
+    case KA_CLOVER: hop_kernel<P, C12, ISIGN, KA_CLOVER><<<grid, block, 0, st>>>(a); break;
+    case KD_TM: hop_kernel<P, C12, ISIGN, KD_TM><<<grid, block, 0, st>>>(a); break;
+    case KA_TM: hop_kernel<P, C12, ISIGN, KA_TM><<<grid, block, 0, st>>>(a); break;
+    case KD_FULL: hop_kernel<P, C12, ISIGN, KD_FULL><<<grid, block, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int P>
+hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int block, hipStream_t st) {
+  if (block <= 0 || block > 256 || (block & 63)) return hipErrorInvalidValue;
+  if (c12) return isign > 0 ? launch_hop3<P, true, 1>(kind, a, block, st) : launch_hop3<P, true, -1>(kind, a, block, st);
+  if constexpr (P != P16) {
+    return isign > 0 ? launch_hop3<P, false, 1>(kind, a, block, st) : launch_hop3<P, false, -1>(kind, a, block, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace qphix_amd

@@ -1,0 +1,46 @@
+"""CPU-side checks of the C-ABI boundary: the library loads, exports every
+entry point include/qphix_amd.h declares, and the host-side validation paths
+reject bad input with the documented status codes without touching a GPU."""
+import ctypes as C
+
+import pytest
+
+import qphix_amd as q
+
+
+def test_library_exports_every_declared_symbol():
+    lib = q.lib()
+    names = q.exported_symbols()
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_geometry_validation_without_gpu():
+    lib = q.lib()
+    h = C.c_void_p()
+    # odd extent -> EDOMAIN (geometry.h:151-154 throws std::domain_error)
+    assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 7, 8), q.F64, 0, C.byref(h)) == q.EDOMAIN
+    assert b"even" in lib.qphix_last_error()
+    # half precision needs two-row compression
+    assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), q.F16, 0, C.byref(h)) == q.EDOMAIN
+    assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), 7, 0, C.byref(h)) == q.EDOMAIN
+    assert lib.qphix_geom_create(None, q.F64, 0, C.byref(h)) == q.EINVAL
+
+
+@pytest.mark.parametrize("prec,c12,expect", [
+    (q.F64, 0, (192, 1152, 576, 1152)),
+    (q.F32, 0, (96, 576, 288, 576)),
+    (q.F32, 1, (96, 384, 288, 576)),
+    (q.F16, 1, (52, 196, 148, 292)),
+])
+def test_field_sizes(prec, c12, expect):
+    g = q.Geometry([8, 8, 8, 8], prec, c12)
+    # bytes per site of spinor / gauge (8 links) / clover / full clover
+    per_site = [b / 2048 for b in g.bytes]
+    assert per_site == list(expect)
+
+
+def test_strerror():
+    lib = q.lib()
+    assert lib.qphix_strerror(q.ENOCONV) == b"solver did not converge"
