@@ -1,0 +1,233 @@
+"""Benchmark: FP32 clover Dslash (A^-1 D psi, one checkerboard) on a 48^3x96
+lattice, reported as algorithmic HBM GB/s, plus the solver comparison of the
+headline metric (mixed-precision clover solve to 1e-10 vs a pure-FP64
+BiCGStab on the same GPU).
+
+One step = one single-parity clover Dslash over all 5,308,416 target sites
+with the inputs resident in HBM. Algorithmic bytes per target site: 8 links
+(12 reals with two-row compression), one source spinor, one output spinor and
+one 72-real clover block: (8*12 + 24 + 24 + 72) * 4 B = 864 B (FP32).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LATT = [48, 48, 48, 96]
+KAPPA = 0.1245
+MASS = 1.0 / (2 * KAPPA) - 4.0   # alpha = 4 + m = 1/(2 kappa)
+CSW = 1.0
+HBM_PEAK_GBS = 8000.0
+BYTES_SITE = {("f32", True): 864, ("f32", False): 1056, ("f64", False): 2112, ("f16", True): 448}
+FLOPS_SITE_CLOVER = 1824
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the clover Dslash kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    path = os.path.join(ROOT, "profiles", "clover_dslash_f32c12_48x48x48x96_pmc.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle's FP64 clover Dslash (hop + A^-1) timed single-threaded on
+    the host on a 16^3x32 sample, in the same algorithmic GB/s."""
+    import numpy as np  # noqa: F401
+    import oracle
+    L = [16, 16, 16, 32]
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    u = oracle.gen_gauge(L, 3, 0.4)
+    clov = oracle.clover_build(L, u, CSW, MASS)
+    inv = oracle.clover_invert(clov[:vh * 72])
+    psi = oracle.gen_spinor(L, 4, 1)
+    hp = oracle.hop_params(-1.0, 1.0, 1.0)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        oracle.clover_apply(inv, oracle.hop(L, u, psi, 1, 0, hp))
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > seconds:
+            break
+    per = dt / n
+    return {"value": round(BYTES_SITE[("f64", False)] * vh / per / 1e9, 3), "unit": "GB/s",
+            "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 clover Dslash (hop + A^-1), 16^3x32, {vh} sites/parity, "
+                      f"{n} applications in {dt:.1f} s ({per * 1e3:.1f} ms each, "
+                      f"{FLOPS_SITE_CLOVER * vh / per / 1e9:.2f} GFLOP/s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
+              "single-GPU with in-kernel periodic boundaries)", file=sys.stderr)
+        sys.exit(2)
+
+    import qphix_amd as q
+
+    vh = LATT[0] * LATT[1] * LATT[2] * LATT[3] // 2
+    t_setup = time.perf_counter()
+    g64 = q.Geometry(LATT, q.F64, False)
+    g32 = q.Geometry(LATT, q.F32, True)
+    g32u = q.Geometry(LATT, q.F32, False)
+    u64 = q.gen_gauge(g64, 3, 0.4)
+    # clover A on cb 1, A^-1 on cb 0 (what the even-odd operator uses for target cb 1)
+    inv64 = q.clover_build(g64, u64, CSW, MASS, 0, q.CLOV_AINV)
+    u32 = q.convert_gauge(g32, g64, u64)
+    u32u = q.convert_gauge(g32u, g64, u64)
+    inv32 = q.convert_clover(g32, g64, inv64)
+    inv32u = q.convert_clover(g32u, g64, inv64)
+    psi = q.gen_spinor(g32, 5, 1)
+    out = g32.spinor()
+    psiu = q.gen_spinor(g32u, 5, 1)
+    outu = g32u.spinor()
+    q.sync()
+    log(f"setup {time.perf_counter() - t_setup:.1f} s")
+
+    D = q.Dslash(g32, t_boundary=-1.0)
+    Du = q.Dslash(g32u, t_boundary=-1.0)
+    timer = q.Timer()
+
+    def run(Dx, o, p, u, inv, k):
+        for _ in range(k):
+            Dx.clov_dslash(o, p, u[0], inv, 1, 0)
+
+    run(D, out, psi, u32, inv32, args.warmup)
+    q.sync()
+    t0 = time.perf_counter()
+    timer.start()
+    run(D, out, psi, u32, inv32, args.steps)
+    ms = timer.stop()
+    wall = time.perf_counter() - t0
+    per_ms = ms / args.steps
+    bytes_launch = BYTES_SITE[("f32", True)] * vh
+    gbs = bytes_launch / (per_ms * 1e-3) / 1e9
+    log(f"clover dslash f32 c12: {per_ms:.4f} ms/launch, {gbs:.1f} GB/s algorithmic")
+
+    # uncompressed gauge, same kernel family (reported alongside)
+    run(Du, outu, psiu, u32u, inv32u, args.warmup)
+    timer.start()
+    run(Du, outu, psiu, u32u, inv32u, args.steps)
+    ms18 = timer.stop() / args.steps
+    gbs18 = BYTES_SITE[("f32", False)] * vh / (ms18 * 1e-3) / 1e9
+    log(f"clover dslash f32 c18: {ms18:.4f} ms/launch, {gbs18:.1f} GB/s algorithmic")
+
+    # copy kernel of known size (FP32 spinor, 96 B/site read + 96 B/site written)
+    # for the profiler's FETCH_SIZE / WRITE_SIZE calibration; not timed
+    for _ in range(5):
+        q.copy(g32, out, psi)
+    q.sync()
+
+    extra = {"ms_per_dslash_c12": round(per_ms, 5), "ms_per_dslash_c18": round(ms18, 5),
+             "gbs_c18": round(gbs18, 1),
+             "gflops_c12": round(FLOPS_SITE_CLOVER * vh / (per_ms * 1e-3) / 1e9, 1)}
+
+    if not args.no_solve:
+        extra["solve"] = solve_compare(q, g64, u64)
+
+    result = {
+        "metric": "clover Dslash GB/s (% HBM peak); solve s to 1e-10, mixed vs FP64",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded U = reunit(exp(0.4 i A)) gauge, Gaussian source, "
+                "clover c_sw=1 built on the GPU in FP64",
+        "config": {"workload": "clover Dslash A^-1 D psi, one parity, 48^3x96, FP32 storage and "
+                               "arithmetic, two-row gauge compression, t antiperiodic",
+                   "lattice": LATT, "sites_per_parity": vh, "kappa": KAPPA, "c_sw": CSW,
+                   "bytes_per_site": BYTES_SITE[("f32", True)]},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles()},
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 thread)...")
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+def solve_compare(q, g64, u64):
+    """Clover solve on the target checkerboard to true residual 1e-10:
+    pure-FP64 BiCGStab vs FP64 defect correction around an FP16-storage
+    BiCGStab (inner relaxation 1e-2)."""
+    import ctypes as C
+    g16 = q.Geometry(LATT, q.F16, True)
+    clov64 = [None, q.clover_build(g64, u64, CSW, MASS, 1, q.CLOV_A)]
+    inv64 = [q.clover_build(g64, u64, CSW, MASS, 0, q.CLOV_AINV), None]
+    u16 = q.convert_gauge(g16, g64, u64)
+    clov16 = [None, q.convert_clover(g16, g64, clov64[1])]
+    inv16 = [q.convert_clover(g16, g64, inv64[0]), None]
+    M64 = q.Operator(g64, q.OP_CLOVER, u64, clov=clov64, invclov=inv64, t_boundary=-1.0)
+    M16 = q.Operator(g16, q.OP_CLOVER, u16, clov=clov16, invclov=inv16, t_boundary=-1.0)
+    b = q.gen_spinor(g64, 7, 1)
+    out = {}
+    # pure FP64 BiCGStab
+    x64 = g64.spinor()
+    s64 = q.InvBiCGStab(M64, 5000)
+    q.sync()
+    t0 = time.perf_counter()
+    st = s64(x64, b, 1e-10)
+    q.sync()
+    t64 = time.perf_counter() - t0
+    out["fp64_bicgstab"] = {"seconds": round(t64, 3), "iters": st.n_iters, "status": st.status}
+    # mixed: FP64 outer Richardson, FP16-storage BiCGStab inner
+    xm = g64.spinor()
+    inner = q.InvBiCGStab(M16, 5000)
+    sm = q.InvRichardsonMultiPrec(M64, inner, 1e-2, 200)
+    q.sync()
+    t0 = time.perf_counter()
+    stm = sm(xm, b, 1e-10)
+    q.sync()
+    tm = time.perf_counter() - t0
+    # true residuals and the difference of the solutions, in FP64
+    r = g64.spinor()
+    M64(r, xm, 1, 1)
+    rn, bn = C.c_double(), C.c_double()
+    q.call("qphix_xmy_norm2", g64.h, r.p, b.p, r.p, C.byref(rn))
+    q.call("qphix_norm2_spinor", g64.h, b.p, C.byref(bn))
+    d = g64.spinor()
+    dn, xn = C.c_double(), C.c_double()
+    q.call("qphix_xmy_norm2", g64.h, d.p, xm.p, x64.p, C.byref(dn))
+    q.call("qphix_norm2_spinor", g64.h, x64.p, C.byref(xn))
+    out["mixed_fp16_richardson"] = {"seconds": round(tm, 3), "outer_iters": stm.n_iters,
+                                    "mv_apps": stm.mv_apps, "status": stm.status,
+                                    "true_rel_resid": float((rn.value / bn.value) ** 0.5),
+                                    "rel_err_vs_fp64": float((dn.value / xn.value) ** 0.5)}
+    out["speedup"] = round(t64 / tm, 3)
+    log(f"solve: fp64 {t64:.2f} s ({st.n_iters} it), mixed {tm:.2f} s, speedup {t64 / tm:.2f}")
+    return out
+
+
+if __name__ == "__main__":
+    main()
