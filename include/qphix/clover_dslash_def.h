@@ -1,0 +1,3 @@
+// Reference file name kept for drop-in includes; the classes live in dslash.h.
+#pragma once
+#include "dslash.h"

@@ -1,0 +1,7 @@
+// Umbrella header of the drop-in QPhiX C++ API for the gfx950 library.
+#pragma once
+#include "geometry.h"
+#include "dslash.h"
+#include "linear_ops.h"
+#include "solvers.h"
+#include "blas_new_c.h"

@@ -1,0 +1,174 @@
+// Exercises the drop-in QPhiX C++ API the way a reference caller does
+// (tests/testClovDslashFull.cc, tests/testDslashFull.cc of JeffersonLab/qphix),
+// compiled with plain g++ against libqphix_amd.so: no HIP headers involved.
+// Prints one "check <name> <value> <limit>" line per check and exits non-zero
+// on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "qphix/qphix.h"
+
+using namespace QPhiX;
+
+static int failures = 0;
+static void check(const char *name, double v, double lim)
+{
+  std::printf("check %s %.3e %.1e %s\n", name, v, lim, v <= lim ? "ok" : "FAIL");
+  if (!(v <= lim)) failures++;
+}
+
+// a user-defined operator: M^+ applied through another operator, so the
+// solvers take their host-driven (non-native) path
+template <typename FT, int V, int S, bool C>
+class Wrapped : public EvenOddLinearOperator<FT, V, S, C> {
+ public:
+  typedef typename Geometry<FT, V, S, C>::FourSpinorBlock Spinor;
+  explicit Wrapped(EvenOddLinearOperator<FT, V, S, C> &M) : M_(M) {}
+  void operator()(Spinor *res, const Spinor *in, int isign, int cb = 1) const override { M_(res, in, isign, cb); }
+  Geometry<FT, V, S, C> &getGeometry() override { return M_.getGeometry(); }
+
+ private:
+  EvenOddLinearOperator<FT, V, S, C> &M_;
+};
+
+int main()
+{
+  const int latt[4] = {8, 8, 8, 8};
+  const double mass = 0.1, csw = 1.0, t_bc = -1.0;
+  typedef Geometry<double, 8, 4, false> G64;
+  typedef Geometry<float, 16, 8, true> G32;
+  G64 geom(latt, 1, 1, 1, 1, 1, 1, 0, 1);
+  G32 geom32(latt, 1, 1, 1, 1, 1, 1, 0, 1);
+
+  // gauge and clover prepared on the device from seeds
+  G64::SU3MatrixBlock *u[2] = {geom.allocCBGauge(), geom.allocCBGauge()};
+  detail::check(qphix_gen_gauge(geom.handle(), u[0], u[1], 11, 0.4), "gen_gauge");
+  G64::CloverBlock *clov = geom.allocCBClov(), *invclov = geom.allocCBClov();
+  detail::check(qphix_clover_build(geom.handle(), u[0], u[1], csw, csw, 4 + mass, 1, QPHIX_CLOV_A, 0, 0, clov), "A");
+  detail::check(qphix_clover_build(geom.handle(), u[0], u[1], csw, csw, 4 + mass, 0, QPHIX_CLOV_AINV, 0, 0, invclov),
+                "Ainv");
+  G32::SU3MatrixBlock *u32[2] = {geom32.allocCBGauge(), geom32.allocCBGauge()};
+  G32::CloverBlock *clov32 = geom32.allocCBClov(), *invclov32 = geom32.allocCBClov();
+  for (int cb = 0; cb < 2; cb++) qphix_convert_gauge(geom32.handle(), u32[cb], geom.handle(), u[cb]);
+  qphix_convert_clover(geom32.handle(), clov32, geom.handle(), clov, 0);
+  qphix_convert_clover(geom32.handle(), invclov32, geom.handle(), invclov, 0);
+
+  G64::FourSpinorBlock *b = geom.allocCBFourSpinor(), *x = geom.allocCBFourSpinor();
+  G64::FourSpinorBlock *r = geom.allocCBFourSpinor(), *t = geom.allocCBFourSpinor();
+  detail::check(qphix_gen_spinor(geom.handle(), b, 5, 1), "gen_spinor");
+  double bnorm;
+  norm2Spinor(bnorm, b, geom, 1);
+
+  // Dslash: A^-1 D psi equals clovMult(A^-1, D psi)
+  {
+    ClovDslash<double, 8, 4, false> D(&geom, t_bc, 1.0, 1.0);
+    Dslash<double, 8, 4, false> W(&geom, t_bc, 1.0, 1.0);
+    D.dslash(r, b, u[0], invclov, 1, 0);
+    W.dslash(t, b, u[0], 1, 0);
+    D.clovMult(x, t, invclov);
+    double n;
+    xmyNorm2Spinor(t, r, x, n, geom, 1);
+    double rn;
+    norm2Spinor(rn, r, geom, 1);
+    check("clov_dslash_vs_clovMult", std::sqrt(n / rn), 1e-14);
+  }
+
+  EvenOddCloverOperator<double, 8, 4, false> M(u, clov, invclov, &geom, t_bc, 1.0, 1.0);
+  // BiCGStab (native path)
+  {
+    zeroSpinor(x, geom, 1);
+    InvBiCGStab<double, 8, 4, false> solver(M, 1000);
+    int it;
+    double rsd;
+    unsigned long fl, mv;
+    solver(x, b, 1e-10, it, rsd, fl, mv, 1, false);
+    M(t, x, 1);
+    double n;
+    xmyNorm2Spinor(r, b, t, n, geom, 1);
+    check("bicgstab_true_resid", std::sqrt(n / bnorm), 1.5e-10);
+  }
+  // CG through a user-defined operator (host-driven loop)
+  {
+    Wrapped<double, 8, 4, false> Mw(M);
+    zeroSpinor(x, geom, 1);
+    InvCG<double, 8, 4, false> cg(Mw, 2000);
+    int it;
+    double rsd;
+    unsigned long fl, mv;
+    cg(x, b, 1e-10, it, rsd, fl, mv, 1, false);
+    M(r, x, 1);
+    M(t, r, -1);
+    double n;
+    xmyNorm2Spinor(r, b, t, n, geom, 1);
+    check("generic_cg_true_resid", std::sqrt(n / bnorm), 1.5e-10);
+  }
+  // mixed precision: FP64 Richardson around FP32 BiCGStab
+  {
+    EvenOddCloverOperator<float, 16, 8, true> M32(u32, clov32, invclov32, &geom32, t_bc, 1.0, 1.0);
+    InvBiCGStab<float, 16, 8, true> inner(M32, 1000);
+    InvRichardsonMultiPrec<double, 8, 4, false, float, 16, 8, true> rich(M, inner, 1e-4, 100);
+    zeroSpinor(x, geom, 1);
+    int it;
+    double rsd;
+    unsigned long fl, mv;
+    rich(x, b, 1e-11, it, rsd, fl, mv, 1, false);
+    M(t, x, 1);
+    double n;
+    xmyNorm2Spinor(r, b, t, n, geom, 1);
+    check("richardson_f64_f32_true_resid", std::sqrt(n / bnorm), 1.5e-11);
+  }
+  // multishift CG
+  {
+    const int ns = 3;
+    double shifts[ns] = {0.001, 0.01, 0.1}, tgt[ns] = {1e-10, 1e-10, 1e-10}, fin[ns];
+    G64::FourSpinorBlock *xs[ns][1];
+    G64::FourSpinorBlock **xp[ns];
+    for (int s = 0; s < ns; s++) {
+      xs[s][0] = geom.allocCBFourSpinor();
+      xp[s] = xs[s];
+    }
+    const G64::FourSpinorBlock *rhs[1] = {b};
+    MInvCG<double, 8, 4, false> mcg(M, 2000, ns);
+    int it;
+    unsigned long fl, mv;
+    mcg(xp, rhs, ns, shifts, tgt, it, fin, fl, mv, 1, false);
+    double worst = 0;
+    for (int s = 0; s < ns; s++) {
+      M(r, xs[s][0], 1);
+      M(t, r, -1);
+      axpy(shifts[s], xs[s][0], t, geom, 1);
+      double n;
+      xmyNorm2Spinor(r, b, t, n, geom, 1);
+      worst = std::fmax(worst, std::sqrt(n / bnorm));
+      geom.free(xs[s][0]);
+    }
+    check("minvcg_worst_shift_true_resid", worst, 2e-10);
+  }
+  // errors map to the reference's exceptions
+  {
+    bool thrown = false;
+    try {
+      const int bad[4] = {8, 8, 8, 7};
+      G64 g2(bad);
+    } catch (const std::domain_error &) {
+      thrown = true;
+    }
+    check("odd_extent_throws_domain_error", thrown ? 0.0 : 1.0, 0.0);
+  }
+  for (int cb = 0; cb < 2; cb++) {
+    geom.free(u[cb]);
+    geom32.free(u32[cb]);
+  }
+  geom.free(clov);
+  geom.free(invclov);
+  geom32.free(clov32);
+  geom32.free(invclov32);
+  geom.free(b);
+  geom.free(x);
+  geom.free(r);
+  geom.free(t);
+  std::printf("%s\n", failures ? "DROPIN FAIL" : "DROPIN OK");
+  return failures ? 1 : 0;
+}

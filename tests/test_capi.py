@@ -44,3 +44,15 @@ def test_field_sizes(prec, c12, expect):
 def test_strerror():
     lib = q.lib()
     assert lib.qphix_strerror(q.ENOCONV) == b"solver did not converge"
+
+
+def test_dropin_program_links():
+    """The drop-in C++ test program links against the library with g++ alone."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "cpp", "dropin_test")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in test program not built")
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libqphix_amd.so" in out and "not found" not in out.split("libqphix_amd.so")[1].split("\n")[0]
