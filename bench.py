@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--hop-config", default=None,
+                    help="block,bz,slabs: fixed hop-kernel launch configuration instead of "
+                         "autotuning (used for profiler runs so every launch is identical)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
@@ -88,6 +91,9 @@ def main():
 
     import qphix_amd as q
 
+    if args.hop_config:
+        blk, bz, sl = (int(v) for v in args.hop_config.split(","))
+        q.call("qphix_tune_config", 2, blk, bz, sl)
     vh = LATT[0] * LATT[1] * LATT[2] * LATT[3] // 2
     t_setup = time.perf_counter()
     g64 = q.Geometry(LATT, q.F64, False)
@@ -141,7 +147,8 @@ def main():
         q.copy(g32, out, psi)
     q.sync()
 
-    extra = {"ms_per_dslash_c12": round(per_ms, 5), "ms_per_dslash_c18": round(ms18, 5),
+    extra = {"tuned": q.tune_summary().strip().splitlines(),
+             "ms_per_dslash_c12": round(per_ms, 5), "ms_per_dslash_c18": round(ms18, 5),
              "gbs_c18": round(gbs18, 1),
              "gflops_c12": round(FLOPS_SITE_CLOVER * vh / (per_ms * 1e-3) / 1e9, 1)}
 
@@ -192,6 +199,12 @@ def solve_compare(q, g64, u64):
     M16 = q.Operator(g16, q.OP_CLOVER, u16, clov=clov16, invclov=inv16, t_boundary=-1.0)
     b = q.gen_spinor(g64, 7, 1)
     out = {}
+    # untimed first applications: the hop kernels of both precisions are
+    # autotuned on first use
+    w64, w16 = g64.spinor(), g16.spinor()
+    M64(w64, b, 1, 1)
+    M16(w16, g16.spinor(), 1, 1)
+    q.sync()
     # pure FP64 BiCGStab
     x64 = g64.spinor()
     s64 = q.InvBiCGStab(M64, 5000)

@@ -62,6 +62,15 @@ int qphix_sync(void);
 /* HIP-event timer on the library stream */
 int qphix_timer_start(void);
 int qphix_timer_stop(double *ms);
+/* Tuning hooks (no reference counterpart: the reference's blocking
+ * parameters By/Bz/Sy/Sz play this role on the CPU). Hop kernels are
+ * autotuned on first use per lattice unless QPHIX_TUNE=0; block is the
+ * default work-group size (64..256, multiple of 64) when tuning is off. */
+int qphix_set_hop_block(int block);
+/* mode 0: no tuning; 1: autotune on first use; 2: force (block, bz, slabs):
+ * bz z-planes per tile, slabs t-slabs (bz = 0: natural site order) */
+int qphix_tune_config(int mode, int block, int bz, int slabs);
+const char *qphix_tune_summary(void);
 
 /* ---- geometry: replaces Geometry<FT,V,S,c12>(latt, By, Bz, NCores, Sy, Sz,
  *      PadXY, PadXYZ, MinCt) (geometry.h:130-235). Blocking / threading

@@ -50,6 +50,7 @@ def lib():
         _lib.qphix_gen_gauge.argtypes = [VP, VP, VP, C.c_uint64, C.c_double]
         _lib.qphix_gen_spinor.argtypes = [VP, VP, C.c_uint64, C.c_int]
         _lib.qphix_op_nflav.restype = C.c_int
+        _lib.qphix_tune_summary.restype = C.c_char_p
     return _lib
 
 
@@ -239,6 +240,10 @@ def convert_clover(g_out, g_in, c_in, full=False):
     out = g_out.full_clover() if full else g_out.clover()
     call("qphix_convert_clover", g_out.h, out.p, g_in.h, c_in.p, int(full))
     return out
+
+
+def tune_summary():
+    return lib().qphix_tune_summary().decode()
 
 
 def sync():

@@ -6,8 +6,10 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -558,6 +560,84 @@ extern "C" int qphix_dslash_destroy(qphix_dslash *d) {
   return QPHIX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// hop-kernel autotuner: per (precision, compression, kernel, isign, lattice) the
+// work-group size and the traversal (natural, or z-block x t tiles per t-slab)
+// are timed once on the first call, into a scratch output so that aliased
+// arguments are never touched twice, and the fastest is kept for the process.
+// All candidates compute bit-identical results; only the order of sites differs.
+// ---------------------------------------------------------------------------
+struct HopCfg {
+  int block, bz, slabs;
+  double ms;
+};
+static std::map<std::string, HopCfg> g_tuned;
+static int g_tune_enabled = -1;         // -1: read QPHIX_TUNE on first use
+static HopCfg g_forced = {0, 0, 0, 0};  // block > 0: every hop uses this configuration
+
+static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block) {
+  switch (g.prec) {
+    case P64: return launch_hop_prec<P64>(kind, g.c12 != 0, isign, a, block, g_stream);
+    case P32: return launch_hop_prec<P32>(kind, g.c12 != 0, isign, a, block, g_stream);
+    default: return launch_hop_prec<P16>(kind, g.c12 != 0, isign, a, block, g_stream);
+  }
+}
+
+static int tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, HopCfg &best) {
+  best.block = g_hop_block;
+  best.bz = 0;
+  best.slabs = 1;
+  best.ms = -1;
+  if (g_forced.block > 0) {
+    best = g_forced;
+    if (best.bz > 0 && (g.L[2] % best.bz || g.L[3] % best.slabs)) {
+      best.bz = 0;
+      best.slabs = 1;
+    }
+    return QPHIX_OK;
+  }
+  if (g_tune_enabled < 0) {
+    const char *e = getenv("QPHIX_TUNE");
+    g_tune_enabled = (e && e[0] == '0') ? 0 : 1;
+  }
+  // small lattices: the traversal order does not matter and tuning would dominate
+  if (!g_tune_enabled || g.vh < (1L << 17)) return QPHIX_OK;
+  std::vector<HopCfg> cand;
+  for (int blk : {128, 256}) {
+    cand.push_back({blk, 0, 1, 0});
+    for (int bz : {2, 4, 6, 8, 12, 16, 24})
+      for (int sl : {8, 16})
+        if (bz <= g.L[2] && g.L[2] % bz == 0 && g.L[3] % sl == 0 && g.L[3] / sl >= 2) cand.push_back({blk, bz, sl, 0});
+  }
+  DevBuf scratch;
+  size_t bytes = g.prec == P64 ? field_bytes<P64>(kSpinorReals, g.vs)
+                 : g.prec == P32 ? field_bytes<P32>(kSpinorReals, g.vs) : field_bytes<P16>(kSpinorReals, g.vs);
+  if (hipMalloc(&scratch.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return QPHIX_OK;  // keep the default configuration
+  }
+  HopArgs a = a0;
+  a.out = scratch.p;
+  for (HopCfg &c : cand) {
+    a.bz = c.bz;
+    a.slabs = c.slabs;
+    if (launch_hop_raw(g, kind, isign, a, c.block) != hipSuccess) {
+      (void)hipGetLastError();
+      c.ms = 1e30;
+      continue;
+    }
+    HIPCHK(hipEventRecord(g_ev0, g_stream));
+    for (int r = 0; r < 3; r++) HIPCHK(launch_hop_raw(g, kind, isign, a, c.block));
+    HIPCHK(hipEventRecord(g_ev1, g_stream));
+    HIPCHK(hipEventSynchronize(g_ev1));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, g_ev0, g_ev1));
+    c.ms = f / 3;
+    if (best.ms < 0 || c.ms < best.ms) best = c;
+  }
+  return QPHIX_OK;
+}
+
 static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *res, const void *psi, const void *u,
                       const void *clov, const void *chi, double coef, double alpha, double mu, double muinv,
                       double rho) {
@@ -596,14 +676,47 @@ static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *
   a.mu = mu;
   a.muinv = muinv;
   a.rho = rho;
-  hipError_t e;
-  switch (g.prec) {
-    case P64: e = launch_hop_prec<P64>(kind, g.c12 != 0, isign, a, g_hop_block, g_stream); break;
-    case P32: e = launch_hop_prec<P32>(kind, g.c12 != 0, isign, a, g_hop_block, g_stream); break;
-    default: e = launch_hop_prec<P16>(kind, g.c12 != 0, isign, a, g_hop_block, g_stream); break;
+  a.bz = 0;
+  a.slabs = 1;
+  char key[128];
+  snprintf(key, sizeof(key), "p%d c%d k%d s%d %dx%dx%dx%d", g.prec, g.c12, kind, isign, g.L[0], g.L[1], g.L[2],
+           g.L[3]);
+  auto it = g_tuned.find(key);
+  if (it == g_tuned.end()) {
+    HopCfg c;
+    TRY(tune_hop(g, kind, isign, a, c));
+    it = g_tuned.emplace(key, c).first;
   }
+  a.bz = it->second.bz;
+  a.slabs = it->second.slabs;
+  hipError_t e = launch_hop_raw(g, kind, isign, a, it->second.block);
   if (e != hipSuccess) return fail(QPHIX_EHIP, "hop kernel launch: %s", hipGetErrorString(e));
   return QPHIX_OK;
+}
+
+// mode 0: no tuning (natural order, default block); 1: autotune on first use;
+// 2: force (block, bz, slabs) for every lattice it divides. Clears the table.
+extern "C" int qphix_tune_config(int mode, int block, int bz, int slabs) {
+  if (mode < 0 || mode > 2) return fail(QPHIX_EDOMAIN, "tune mode %d", mode);
+  if (mode == 2 && (block <= 0 || block > 256 || (block & 63) || bz < 0 || (bz > 0 && slabs < 1)))
+    return fail(QPHIX_EDOMAIN, "forced configuration block %d bz %d slabs %d", block, bz, slabs);
+  g_tuned.clear();
+  g_tune_enabled = mode == 1 ? 1 : 0;
+  g_forced = mode == 2 ? HopCfg{block, bz, bz > 0 ? slabs : 1, 0} : HopCfg{0, 0, 0, 0};
+  return QPHIX_OK;
+}
+
+// text summary of the tuned hop-kernel launch configurations
+extern "C" const char *qphix_tune_summary(void) {
+  static std::string out;
+  out.clear();
+  for (auto &kv : g_tuned) {
+    char line[256];
+    snprintf(line, sizeof(line), "%s: block %d bz %d slabs %d %.4f ms\n", kv.first.c_str(), kv.second.block,
+             kv.second.bz, kv.second.slabs, kv.second.ms);
+    out += line;
+  }
+  return out.c_str();
 }
 
 extern "C" int qphix_wilson_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign, int cb) {
@@ -1157,6 +1270,7 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
   if (s->kind != S_MINVCG) return fail(QPHIX_EINVAL, "not a multishift solver");
   if (n_shift < 1 || n_shift > s->max_shifts)
     return fail(QPHIX_EDOMAIN, "n_shift = %d exceeds MaxShifts = %d", n_shift, s->max_shifts);
+  if (n_shift > kMaxShifts) return fail(QPHIX_EDOMAIN, "n_shift = %d exceeds the library limit %d", n_shift, kMaxShifts);
   qphix_op *M = s->M;
   const qphix_geom *g = M->g;
   const int nf = M->nflav;
@@ -1197,17 +1311,44 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
     beta[q] = b / (1.0 - shifts[q] * b);
     for (int f = 0; f < nf; f++) TRY(qphix_axy(g, -beta[q], rhs[f], x[q * nf + f]));
   }
+  // one fused sweep per iteration updates p0 and every shift's p_s, and applies
+  // the x_s update left pending by the previous iteration (minvcg.h:298-363)
+  auto fused = [&](int do_p0, double a, unsigned mask_p, unsigned mask_x, const double *as) -> int {
+    for (int f = 0; f < nf; f++) {
+      MShiftArgs m;
+      memset(&m, 0, sizeof(m));
+      m.r = r[f];
+      m.p0 = p0[f];
+      m.a = a;
+      m.do_p0 = do_p0;
+      m.nshift = n_shift;
+      m.mask_p = mask_p;
+      m.mask_x = mask_x;
+      for (int q = 0; q < n_shift; q++) {
+        m.p[q] = p[q][f];
+        m.x[q] = x[q * nf + f];
+        m.zeta[q] = zeta[q];
+        m.as[q] = as ? as[q] : 0.0;
+        m.beta[q] = beta[q];
+      }
+      HIPCHK(mshift_launch(g->d, m, g_stream));
+    }
+    return QPHIX_OK;
+  };
   bool convP = false;
   int k;
+  unsigned pending_x = 0;
+  std::vector<double> as(n_shift, 0.0);
   *n_iters = 0;
   for (k = 1; k < s->max_iters && !convP; ++k) {
     const double a = c / cp;
-    for (int f = 0; f < nf; f++) TRY(qphix_aypx(g, a, r[f], p0[f]));
+    unsigned mask_p = 0;
     for (int q = 0; q < n_shift; q++)
       if (!conv[q]) {
-        const double as = a * zeta[q] * beta[q] / (zeta_prev[q] * b);
-        for (int f = 0; f < nf; f++) TRY(qphix_axpby(g, zeta[q], r[f], as, p[q][f]));
+        as[q] = a * zeta[q] * beta[q] / (zeta_prev[q] * b);
+        mask_p |= 1u << q;
       }
+    TRY(fused(1, a, mask_p, pending_x, as.data()));
     cp = c;
     const double bp = b;
     TRY(op_apply_v(M, mp, cv(p0), isign, cb));
@@ -1222,6 +1363,7 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
       c += t;
     }
     convP = true;
+    pending_x = 0;
     for (int q = 0; q < n_shift; q++) {
       if (!conv[q]) {
         const double z0 = zeta[q], z1 = zeta_prev[q];
@@ -1229,7 +1371,7 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
         zeta[q] = z0 * z1 * bp;
         zeta[q] /= (b * a * (z1 - z0) + z1 * bp * (1.0 - shifts[q] * b));
         beta[q] = b * zeta[q] / z0;
-        for (int f = 0; f < nf; f++) TRY(qphix_axpy(g, -beta[q], p[q][f], x[q * nf + f]));
+        pending_x |= 1u << q;
         const double css = c * zeta[q] * zeta[q];
         if (verbose) printf("MINVCG: iter=%d shift=%d css=%16.8e target=%16.8e\n", k, q, css, rsd_sq[q]);
         conv[q] = css < rsd_sq[q];
@@ -1239,6 +1381,7 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
     }
     *n_iters = k;
   }
+  if (pending_x) TRY(fused(0, 0.0, 0u, pending_x, nullptr));
   if (site_flops) *site_flops = 0;
   if (mv_apps) *mv_apps = mv;
   for (void *w : s->work) qphix_free(w);

@@ -37,26 +37,26 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
   T x[N], y[N];
   const T a0 = (T)a.a[0], a1 = (T)a.a[1];
   if constexpr (OP == B_COPY) {
-    load_spinor<P>(a.x, vs, s, x);
-    store_spinor<P>(a.o0, vs, s, x);
+    load_spinor<P, true>(a.x, vs, s, x);
+    store_spinor<P, true>(a.o0, vs, s, x);
   } else if constexpr (OP == B_ZERO) {
 #pragma unroll
     for (int k = 0; k < N; k++) x[k] = 0;
-    store_spinor<P>(a.o0, vs, s, x);
+    store_spinor<P, true>(a.o0, vs, s, x);
   } else if constexpr (OP == B_AXY) {  // y = a x
-    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P, true>(a.x, vs, s, x);
 #pragma unroll
     for (int k = 0; k < N; k++) y[k] = a0 * x[k];
-    store_spinor<P>(a.o0, vs, s, y);
+    store_spinor<P, true>(a.o0, vs, s, y);
   } else if constexpr (OP == B_AX) {  // x = a x
-    load_spinor<P>(a.o0, vs, s, x);
+    load_spinor<P, true>(a.o0, vs, s, x);
 #pragma unroll
     for (int k = 0; k < N; k++) x[k] = a0 * x[k];
-    store_spinor<P>(a.o0, vs, s, x);
+    store_spinor<P, true>(a.o0, vs, s, x);
   } else if constexpr (OP == B_YPEQX || OP == B_YMEQX || OP == B_AYPX || OP == B_AXPY || OP == B_AXPBY ||
                        OP == B_AXPYNORM2 || OP == B_XMY2NORM2 || OP == B_BICG_XMY) {
-    load_spinor<P>(a.x, vs, s, x);
-    load_spinor<P>(a.o0, vs, s, y);
+    load_spinor<P, true>(a.x, vs, s, x);
+    load_spinor<P, true>(a.o0, vs, s, y);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       if constexpr (OP == B_YPEQX) y[k] = y[k] + x[k];
@@ -67,46 +67,46 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       if constexpr (OP == B_BICG_XMY) y[k] = x[k] - y[k];
     }
     if constexpr (OP == B_AXPYNORM2 || OP == B_XMY2NORM2) red[0] += sq(y);
-    store_spinor<P>(a.o0, vs, s, y);
+    store_spinor<P, true>(a.o0, vs, s, y);
   } else if constexpr (OP == B_NORM2) {
-    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P, true>(a.x, vs, s, x);
     red[0] += sq(x);
   } else if constexpr (OP == B_XMYNORM2) {  // o0 = x - y
-    load_spinor<P>(a.x, vs, s, x);
-    load_spinor<P>(a.y, vs, s, y);
+    load_spinor<P, true>(a.x, vs, s, x);
+    load_spinor<P, true>(a.y, vs, s, y);
 #pragma unroll
     for (int k = 0; k < N; k++) x[k] = x[k] - y[k];
     red[0] += sq(x);
-    store_spinor<P>(a.o0, vs, s, x);
+    store_spinor<P, true>(a.o0, vs, s, x);
   } else if constexpr (OP == B_RMAMMP) {  // r -= a mmp ; x += a p
     T r[N], xx[N];
-    load_spinor<P>(a.x, vs, s, x);   // mmp
-    load_spinor<P>(a.o0, vs, s, r);  // r
+    load_spinor<P, true>(a.x, vs, s, x);   // mmp
+    load_spinor<P, true>(a.o0, vs, s, r);  // r
 #pragma unroll
     for (int k = 0; k < N; k++) r[k] = r[k] - a0 * x[k];
     red[0] += sq(r);
-    store_spinor<P>(a.o0, vs, s, r);
-    load_spinor<P>(a.y, vs, s, y);    // p
-    load_spinor<P>(a.o1, vs, s, xx);  // x
+    store_spinor<P, true>(a.o0, vs, s, r);
+    load_spinor<P, true>(a.y, vs, s, y);    // p
+    load_spinor<P, true>(a.o1, vs, s, xx);  // x
 #pragma unroll
     for (int k = 0; k < N; k++) xx[k] = xx[k] + a0 * y[k];
-    store_spinor<P>(a.o1, vs, s, xx);
+    store_spinor<P, true>(a.o1, vs, s, xx);
   } else if constexpr (OP == B_RICH) {  // x += dx ; r -= dr
     T t[N];
-    load_spinor<P>(a.x, vs, s, x);   // dx
-    load_spinor<P>(a.o0, vs, s, t);  // x
+    load_spinor<P, true>(a.x, vs, s, x);   // dx
+    load_spinor<P, true>(a.o0, vs, s, t);  // x
 #pragma unroll
     for (int k = 0; k < N; k++) t[k] = t[k] + x[k];
-    store_spinor<P>(a.o0, vs, s, t);
-    load_spinor<P>(a.y, vs, s, y);   // dr
-    load_spinor<P>(a.o1, vs, s, t);  // r
+    store_spinor<P, true>(a.o0, vs, s, t);
+    load_spinor<P, true>(a.y, vs, s, y);   // dr
+    load_spinor<P, true>(a.o1, vs, s, t);  // r
 #pragma unroll
     for (int k = 0; k < N; k++) t[k] = t[k] - y[k];
     red[0] += sq(t);
-    store_spinor<P>(a.o1, vs, s, t);
+    store_spinor<P, true>(a.o1, vs, s, t);
   } else if constexpr (OP == B_CDOT || OP == B_CDOTNORM) {  // <x, y> = sum conj(x) y
-    load_spinor<P>(a.x, vs, s, x);
-    load_spinor<P>(a.y, vs, s, y);
+    load_spinor<P, true>(a.x, vs, s, x);
+    load_spinor<P, true>(a.y, vs, s, y);
     double re = 0, im = 0;
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
@@ -118,9 +118,9 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
     if constexpr (OP == B_CDOTNORM) red[2] += sq(x);
   } else if constexpr (OP == B_PUPD) {  // p = r + beta (p - omega v); x=r, y=v, o0=p, a = beta, omega
     T v[N];
-    load_spinor<P>(a.x, vs, s, x);
-    load_spinor<P>(a.y, vs, s, v);
-    load_spinor<P>(a.o0, vs, s, y);
+    load_spinor<P, true>(a.x, vs, s, x);
+    load_spinor<P, true>(a.y, vs, s, v);
+    load_spinor<P, true>(a.o0, vs, s, y);
     const T br = a0, bi = a1, wr = (T)a.a[2], wi = (T)a.a[3];
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
@@ -129,10 +129,10 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       y[k] = x[k] + (br * tr - bi * ti);
       y[k + 1] = x[k + 1] + (br * ti + bi * tr);
     }
-    store_spinor<P>(a.o0, vs, s, y);
+    store_spinor<P, true>(a.o0, vs, s, y);
   } else if constexpr (OP == B_SUPD || OP == B_CAXPY) {  // s -= alpha v ; y += alpha x
-    load_spinor<P>(a.x, vs, s, x);
-    load_spinor<P>(a.o0, vs, s, y);
+    load_spinor<P, true>(a.x, vs, s, x);
+    load_spinor<P, true>(a.o0, vs, s, y);
     const T sg = (OP == B_SUPD) ? (T)-1 : (T)1;
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
@@ -140,20 +140,20 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       y[k] = y[k] + sg * pr;
       y[k + 1] = y[k + 1] + sg * pi;
     }
-    store_spinor<P>(a.o0, vs, s, y);
+    store_spinor<P, true>(a.o0, vs, s, y);
   } else if constexpr (OP == B_RXUPD) {  // x += omega r + alpha p ; r -= omega t  (x=t, y=p, o0=x, o1=r)
     T r[N], xx[N];
     const T wr = a0, wi = a1, ar = (T)a.a[2], ai = (T)a.a[3];
-    load_spinor<P>(a.o1, vs, s, r);
-    load_spinor<P>(a.y, vs, s, y);
-    load_spinor<P>(a.o0, vs, s, xx);
+    load_spinor<P, true>(a.o1, vs, s, r);
+    load_spinor<P, true>(a.y, vs, s, y);
+    load_spinor<P, true>(a.o0, vs, s, xx);
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
       xx[k] = xx[k] + (wr * r[k] - wi * r[k + 1]) + (ar * y[k] - ai * y[k + 1]);
       xx[k + 1] = xx[k + 1] + (wr * r[k + 1] + wi * r[k]) + (ar * y[k + 1] + ai * y[k]);
     }
-    store_spinor<P>(a.o0, vs, s, xx);
-    load_spinor<P>(a.x, vs, s, x);
+    store_spinor<P, true>(a.o0, vs, s, xx);
+    load_spinor<P, true>(a.x, vs, s, x);
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
       const T tr = wr * x[k] - wi * x[k + 1], ti = wr * x[k + 1] + wi * x[k];
@@ -161,12 +161,12 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       r[k + 1] = r[k + 1] - ti;
     }
     red[0] += sq(r);
-    store_spinor<P>(a.o1, vs, s, r);
+    store_spinor<P, true>(a.o1, vs, s, r);
   } else if constexpr (OP == B_MRUPD) {  // x += a r ; r -= a Mr   (x = Mr, o0 = x, o1 = r)
     T r[N], xx[N];
-    load_spinor<P>(a.o1, vs, s, r);
-    load_spinor<P>(a.o0, vs, s, xx);
-    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P, true>(a.o1, vs, s, r);
+    load_spinor<P, true>(a.o0, vs, s, xx);
+    load_spinor<P, true>(a.x, vs, s, x);
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
       xx[k] = xx[k] + (a0 * r[k] - a1 * r[k + 1]);
@@ -174,22 +174,22 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       r[k] = r[k] - (a0 * x[k] - a1 * x[k + 1]);
       r[k + 1] = r[k + 1] - (a0 * x[k + 1] + a1 * x[k]);
     }
-    store_spinor<P>(a.o0, vs, s, xx);
-    store_spinor<P>(a.o1, vs, s, r);
+    store_spinor<P, true>(a.o0, vs, s, xx);
+    store_spinor<P, true>(a.o1, vs, s, r);
   } else if constexpr (OP == B_TWM) {  // y = (a0 + i a1 g5) x
-    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P, true>(a.x, vs, s, x);
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
       const T g = (k < 12) ? a1 : -a1;
       y[k] = a0 * x[k] - g * x[k + 1];
       y[k + 1] = a0 * x[k + 1] + g * x[k];
     }
-    store_spinor<P>(a.o0, vs, s, y);
+    store_spinor<P, true>(a.o0, vs, s, y);
   } else if constexpr (OP == B_TWM2) {  // (a0 + i a1 g5 tau3 + eps tau1): x=up, y=dn, o0, o1
     T u[N], d[N], ou[N], od[N];
     const T e = (T)a.a[2];
-    load_spinor<P>(a.x, vs, s, u);
-    load_spinor<P>(a.y, vs, s, d);
+    load_spinor<P, true>(a.x, vs, s, u);
+    load_spinor<P, true>(a.y, vs, s, d);
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
       const T g = (k < 12) ? a1 : -a1;
@@ -198,11 +198,11 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       od[k] = a0 * d[k] + g * d[k + 1] + e * u[k];
       od[k + 1] = a0 * d[k + 1] - g * d[k] + e * u[k + 1];
     }
-    store_spinor<P>(a.o0, vs, s, ou);
-    store_spinor<P>(a.o1, vs, s, od);
+    store_spinor<P, true>(a.o0, vs, s, ou);
+    store_spinor<P, true>(a.o1, vs, s, od);
   } else if constexpr (OP == B_CLOVMULT || OP == B_FULLMULT) {  // o0 = A x  (y = clover)
     cx<T> in[4][3], out[4][3];
-    load_spinor<P>(a.x, vs, s, x);
+    load_spinor<P, true>(a.x, vs, s, x);
     to_spinor(x, in);
     if constexpr (OP == B_CLOVMULT) {
       clover_block_apply<P, 0>(a.y, vs, s, in, out);
@@ -212,12 +212,12 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       full_block_apply<P, 1>(a.y, vs, s, in, out);
     }
     from_spinor(out, y);
-    store_spinor<P>(a.o0, vs, s, y);
+    store_spinor<P, true>(a.o0, vs, s, y);
   } else if constexpr (OP == B_TWOFLAV_INVCLOV) {
     // o0 = Fu x + C y ; o1 = Fd y + C x   (x = psi_up, y = psi_dn, z = Fu, w = Fd, o2 = C)
     cx<T> pu[4][3], pd[4][3], r[4][3], c[4][3];
-    load_spinor<P>(a.x, vs, s, x);
-    load_spinor<P>(a.y, vs, s, y);
+    load_spinor<P, true>(a.x, vs, s, x);
+    load_spinor<P, true>(a.y, vs, s, y);
     to_spinor(x, pu);
     to_spinor(y, pd);
     full_block_apply<P, 0>(a.z, vs, s, pu, r);
@@ -230,7 +230,7 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       for (int cc = 0; cc < 3; cc++) r[sp][cc] = r[sp][cc] + c[sp][cc];
     T o[N];
     from_spinor(r, o);
-    store_spinor<P>(a.o0, vs, s, o);
+    store_spinor<P, true>(a.o0, vs, s, o);
     full_block_apply<P, 0>(a.w, vs, s, pd, r);
     full_block_apply<P, 1>(a.w, vs, s, pd, r);
     clover_block_apply<P, 0>(a.o2, vs, s, pu, c);
@@ -240,7 +240,7 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
 #pragma unroll
       for (int cc = 0; cc < 3; cc++) r[sp][cc] = r[sp][cc] + c[sp][cc];
     from_spinor(r, o);
-    store_spinor<P>(a.o1, vs, s, o);
+    store_spinor<P, true>(a.o1, vs, s, o);
   }
 }
 
@@ -328,6 +328,54 @@ hipError_t blas_launch(const GeomDesc &g, int op, const BlasArgs &a, double *par
     case P16: return launch_prec<P16>(g, op, a, partials, (int)nb, st);
   }
   return hipErrorInvalidValue;
+}
+
+template <int P>
+__global__ void __launch_bounds__(kBlasThreads) k_mshift(const MShiftArgs m, size_t vs, long vh) {
+  typedef typename PT<P>::calc T;
+  constexpr int N = kSpinorReals;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < vh; s += stride) {
+    T r[N], v[N], w[N];
+    load_spinor<P, true>(m.r, vs, s, r);
+    if (m.do_p0) {
+      load_spinor<P, true>(m.p0, vs, s, v);
+      const T a = (T)m.a;
+#pragma unroll
+      for (int k = 0; k < N; k++) v[k] = r[k] + a * v[k];
+      store_spinor<P, true>(m.p0, vs, s, v);
+    }
+    for (int q = 0; q < m.nshift; q++) {
+      const unsigned bit = 1u << q;
+      if (!((m.mask_x | m.mask_p) & bit)) continue;
+      load_spinor<P, true>(m.p[q], vs, s, v);
+      if (m.mask_x & bit) {
+        load_spinor<P, true>(m.x[q], vs, s, w);
+        const T b = (T)m.beta[q];
+#pragma unroll
+        for (int k = 0; k < N; k++) w[k] = w[k] - b * v[k];
+        store_spinor<P, true>(m.x[q], vs, s, w);
+      }
+      if (m.mask_p & bit) {
+        const T z = (T)m.zeta[q], as = (T)m.as[q];
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = z * r[k] + as * v[k];
+        store_spinor<P, true>(m.p[q], vs, s, v);
+      }
+    }
+  }
+}
+
+hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st) {
+  long nb = (g.vh + kBlasThreads - 1) / kBlasThreads;
+  if (nb > kMaxBlasBlocks) nb = kMaxBlasBlocks;
+  switch (g.prec) {
+    case P64: k_mshift<P64><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
+    case P32: k_mshift<P32><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
+    case P16: k_mshift<P16><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t blas_finalize(const double *partials, int nblocks, int nred, double *out, hipStream_t st) {

@@ -1,14 +1,17 @@
 // Shared device/host definitions for the gfx950 Dslash library.
 //
-// Field layout in HBM (site-major structure of arrays):
-//   A field is an array of W "words" of 16 bytes per site. Word w of
-//   checkerboard site s lives at byte offset (w * vs + s) * 16, where vs is the
-//   per-plane stride (Vh rounded up to a multiple of 256 sites). Consecutive
-//   lanes of a wavefront read consecutive sites, so each word load is one
-//   fully coalesced 1 KiB transaction per wavefront.
+// Field layout in HBM (site-blocked structure of arrays, "AoSoA-64"):
+//   A field is W "words" of 16 bytes per site. Sites are grouped in blocks of
+//   64 consecutive checkerboard sites; within a block the W words are stored
+//   word-major, so word w of site s lives at 16-byte index
+//   ((s / 64) * W + w) * 64 + s % 64. One wavefront (64 lanes, 64 consecutive
+//   sites) reads each word as one coalesced 1 KiB transaction, and all W words
+//   of its sites lie in one contiguous W KiB region, which keeps the number of
+//   DRAM pages and TLB entries a wavefront touches small.
 //   A word holds 2 doubles (F64), 4 floats (F32) or 8 halves (F16). F16 fields
-//   carry one float scale per site after the last word plane
+//   carry one float scale per site after the last block
 //   (byte offset W * vs * 16 + s * 4); the stored halves are value / scale.
+//   vs (sites rounded up to a multiple of 256) sizes the allocation.
 //
 // Per-site real counts (same real order as the canonical host layout):
 //   spinor        24   [spin 4][colour 3][re,im]
@@ -64,52 +67,61 @@ template <int P> __host__ __device__ inline size_t field_bytes(int nreal, size_t
 // ---------------------------------------------------------------------------
 // word loads / stores
 // ---------------------------------------------------------------------------
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// 16-byte index of word w of site s in a field of nw words per site
+__host__ __device__ inline size_t widx(long s, int w, int nw) {
+  return ((size_t)(s >> 6) * nw + w) * 64 + (size_t)(s & 63);
+}
+
+// NT selects non-temporal (streaming) loads / stores for data touched once
 template <int P> struct Word;
 
 template <> struct Word<P64> {
-  __device__ static inline void load(const void *base, size_t vs, long s, int w, double *o) {
-    const double2 v = reinterpret_cast<const double2 *>(base)[(size_t)w * vs + s];
+  template <bool NT> __device__ static inline void load(const void *base, size_t i, double *o) {
+    const d2_t *p = reinterpret_cast<const d2_t *>(base) + i;
+    const d2_t v = NT ? __builtin_nontemporal_load(p) : *p;
     o[0] = v.x;
     o[1] = v.y;
   }
-  __device__ static inline void store(void *base, size_t vs, long s, int w, const double *i) {
-    double2 v;
-    v.x = i[0];
-    v.y = i[1];
-    reinterpret_cast<double2 *>(base)[(size_t)w * vs + s] = v;
+  template <bool NT> __device__ static inline void store(void *base, size_t i, const double *in) {
+    d2_t v = {in[0], in[1]};
+    d2_t *p = reinterpret_cast<d2_t *>(base) + i;
+    if (NT) __builtin_nontemporal_store(v, p); else *p = v;
   }
 };
 
 template <> struct Word<P32> {
-  __device__ static inline void load(const void *base, size_t vs, long s, int w, float *o) {
-    const float4 v = reinterpret_cast<const float4 *>(base)[(size_t)w * vs + s];
+  template <bool NT> __device__ static inline void load(const void *base, size_t i, float *o) {
+    const f4_t *p = reinterpret_cast<const f4_t *>(base) + i;
+    const f4_t v = NT ? __builtin_nontemporal_load(p) : *p;
     o[0] = v.x;
     o[1] = v.y;
     o[2] = v.z;
     o[3] = v.w;
   }
-  __device__ static inline void store(void *base, size_t vs, long s, int w, const float *i) {
-    float4 v;
-    v.x = i[0];
-    v.y = i[1];
-    v.z = i[2];
-    v.w = i[3];
-    reinterpret_cast<float4 *>(base)[(size_t)w * vs + s] = v;
+  template <bool NT> __device__ static inline void store(void *base, size_t i, const float *in) {
+    f4_t v = {in[0], in[1], in[2], in[3]};
+    f4_t *p = reinterpret_cast<f4_t *>(base) + i;
+    if (NT) __builtin_nontemporal_store(v, p); else *p = v;
   }
 };
 
 template <> struct Word<P16> {
   // loads raw (unscaled) values; callers multiply by the site scale
-  __device__ static inline void load(const void *base, size_t vs, long s, int w, float *o) {
-    const half8_t v = reinterpret_cast<const half8_t *>(base)[(size_t)w * vs + s];
+  template <bool NT> __device__ static inline void load(const void *base, size_t i, float *o) {
+    const half8_t *p = reinterpret_cast<const half8_t *>(base) + i;
+    const half8_t v = NT ? __builtin_nontemporal_load(p) : *p;
 #pragma unroll
     for (int k = 0; k < 8; k++) o[k] = (float)v[k];
   }
-  __device__ static inline void store(void *base, size_t vs, long s, int w, const float *i) {
+  template <bool NT> __device__ static inline void store(void *base, size_t i, const float *in) {
     half8_t v;
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = (_Float16)i[k];
-    reinterpret_cast<half8_t *>(base)[(size_t)w * vs + s] = v;
+    for (int k = 0; k < 8; k++) v[k] = (_Float16)in[k];
+    half8_t *p = reinterpret_cast<half8_t *>(base) + i;
+    if (NT) __builtin_nontemporal_store(v, p); else *p = v;
   }
 };
 
@@ -119,11 +131,11 @@ template <int P> __device__ inline float site_scale(const void *base, size_t vs,
 }
 
 // Load reals [W0*per_word, (W0+NW)*per_word) of a field with NWT words per site.
-template <int P, int NWT, int W0, int NW>
+template <int P, int NWT, int W0, int NW, bool NT = false>
 __device__ inline void load_range(const void *base, size_t vs, long s, typename PT<P>::calc *o) {
   typedef typename PT<P>::calc T;
 #pragma unroll
-  for (int w = 0; w < NW; w++) Word<P>::load(base, vs, s, W0 + w, o + w * PT<P>::per_word);
+  for (int w = 0; w < NW; w++) Word<P>::template load<NT>(base, widx(s, W0 + w, NWT), o + w * PT<P>::per_word);
   if constexpr (PT<P>::scaled) {
     T sc = site_scale<P>(base, vs, s, NWT);
 #pragma unroll
@@ -132,7 +144,7 @@ __device__ inline void load_range(const void *base, size_t vs, long s, typename 
 }
 
 // Store all NWT words of a site (computes the F16 per-site scale).
-template <int P, int NWT>
+template <int P, int NWT, bool NT = false>
 __device__ inline void store_all(void *base, size_t vs, long s, const typename PT<P>::calc *v) {
   constexpr int N = NWT * PT<P>::per_word;
   if constexpr (PT<P>::scaled) {
@@ -144,22 +156,22 @@ __device__ inline void store_all(void *base, size_t vs, long s, const typename P
 #pragma unroll
     for (int k = 0; k < N; k++) tmp[k] = v[k] * inv;
 #pragma unroll
-    for (int w = 0; w < NWT; w++) Word<P>::store(base, vs, s, w, tmp + w * PT<P>::per_word);
+    for (int w = 0; w < NWT; w++) Word<P>::template store<NT>(base, widx(s, w, NWT), tmp + w * PT<P>::per_word);
     float *sc = reinterpret_cast<float *>(reinterpret_cast<char *>(base) + (size_t)NWT * vs * 16);
     sc[s] = m;
   } else {
 #pragma unroll
-    for (int w = 0; w < NWT; w++) Word<P>::store(base, vs, s, w, v + w * PT<P>::per_word);
+    for (int w = 0; w < NWT; w++) Word<P>::template store<NT>(base, widx(s, w, NWT), v + w * PT<P>::per_word);
   }
 }
 
-template <int P>
+template <int P, bool NT = false>
 __device__ inline void load_spinor(const void *base, size_t vs, long s, typename PT<P>::calc *o) {
-  load_range<P, words_of<P>(kSpinorReals), 0, words_of<P>(kSpinorReals)>(base, vs, s, o);
+  load_range<P, words_of<P>(kSpinorReals), 0, words_of<P>(kSpinorReals), NT>(base, vs, s, o);
 }
-template <int P>
+template <int P, bool NT = false>
 __device__ inline void store_spinor(void *base, size_t vs, long s, const typename PT<P>::calc *v) {
-  store_all<P, words_of<P>(kSpinorReals)>(base, vs, s, v);
+  store_all<P, words_of<P>(kSpinorReals), NT>(base, vs, s, v);
 }
 
 // ---------------------------------------------------------------------------

@@ -51,4 +51,21 @@ hipError_t blas_launch(const GeomDesc &g, int op, const BlasArgs &a, double *par
 hipError_t blas_finalize(const double *partials, int nblocks, int nred, double *out, hipStream_t st);
 constexpr int kMaxBlasBlocks = 2048;
 
+// Multishift CG vector updates for every shift in one sweep (minvcg.h:298-363):
+//   x_s -= beta_s p_s      for s in mask_x   (uses p_s before its update)
+//   p0   = r + a p0        if do_p0
+//   p_s  = zeta_s r + as_s p_s   for s in mask_p
+constexpr int kMaxShifts = 32;
+struct MShiftArgs {
+  const void *r;
+  void *p0;
+  void *p[kMaxShifts];
+  void *x[kMaxShifts];
+  double a;
+  double zeta[kMaxShifts], as[kMaxShifts], beta[kMaxShifts];
+  unsigned mask_x, mask_p;
+  int do_p0, nshift;
+};
+hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st);
+
 }  // namespace qphix_amd

@@ -36,6 +36,10 @@ struct HopArgs {
   double tbc_r[4], tbc_i[4];
   int tbc_mask;
   double alpha, mu, muinv, rho;
+  // traversal: 0 = natural site order; otherwise each of `slabs` t-slabs is
+  // walked z-block by z-block (bz planes), t innermost, so consecutive work-
+  // groups of one XCD reuse the source spinor planes they share
+  int bz, slabs;
 };
 
 // block -> lattice-tile remap so that each XCD (8 per device, round-robin
@@ -130,19 +134,19 @@ __device__ inline void hop_dim(const HopArgs &a, long s, long sb, long sf, typen
     T lv[GL::link_reals];
     // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
     load_spinor<P>(a.psi, a.vs, sb, sp);
-    load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words>(a.u, a.vs, s, lv);
+    load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
     apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
     // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
     load_spinor<P>(a.psi, a.vs, sf, sp);
-    load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words>(a.u, a.vs, s, lv);
+    load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
     apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tbc, tr, ti, acc);
   } else {
     T lv[2 * GL::link_reals];
-    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words>(a.u, a.vs, s, lv);
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
     load_spinor<P>(a.psi, a.vs, sb, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
@@ -160,10 +164,10 @@ template <int P, int B> __device__ inline void load_clover_block(const void *c, 
                                                                  typename PT<P>::calc *o) {
   constexpr int nw = words_of<P>(kCloverReals);
   if constexpr ((36 % PT<P>::per_word) == 0) {
-    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word>(c, vs, s, o);
+    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word, true>(c, vs, s, o);
   } else {
     typename PT<P>::calc all[kCloverReals];
-    load_range<P, nw, 0, nw>(c, vs, s, all);
+    load_range<P, nw, 0, nw, true>(c, vs, s, all);
 #pragma unroll
     for (int k = 0; k < 36; k++) o[k] = all[36 * B + k];
   }
@@ -217,7 +221,7 @@ __device__ inline void full_block_apply(const void *fcl, size_t vs, long s, cons
   constexpr int nw = words_of<P>(kFullCloverReals);
   constexpr int bw = nw / 2;
   T m[72];
-  load_range<P, nw, B * bw, bw>(fcl, vs, s, m);
+  load_range<P, nw, B * bw, bw, true>(fcl, vs, s, m);
   cx<T> v[6], w[6];
 #pragma unroll
   for (int i = 0; i < 6; i++) v[i] = in[2 * B + i / 3][i % 3];
@@ -239,8 +243,22 @@ template <typename T> __device__ inline void add_ig5(T g, const cx<T> (&in)[4][3
 template <int P, bool C12, int ISIGN, int KIND>
 __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   typedef typename PT<P>::calc T;
-  const long s = xcd_site(blockDim.x);
-  if (s >= a.vh) return;
+  const long k = xcd_site(blockDim.x);
+  if (k >= a.vh) return;
+  long s = k;
+  if (a.bz > 0) {
+    const int plane = a.nxh * a.ny;
+    const int nts = a.nt / a.slabs, nzb = a.nz / a.bz;
+    long q = k / plane;
+    const int p = (int)(k - q * plane);
+    const int zi = (int)(q % a.bz);
+    q /= a.bz;
+    const int ti = (int)(q % nts);
+    q /= nts;
+    const int zb = (int)(q % nzb);
+    const int slab = (int)(q / nzb);
+    s = ((long)(slab * nts + ti) * a.nz + zb * a.bz + zi) * plane + p;
+  }
 
   // coordinates of the target site
   int r = (int)s;
@@ -272,7 +290,7 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   if constexpr (KIND == KA_WILSON || KIND == KA_CLOVER || KIND == KA_TM) {
     T cv[kSpinorReals];
     cx<T> chi[4][3];
-    load_spinor<P>(a.chi, a.vs, s, cv);
+    load_spinor<P, true>(a.chi, a.vs, s, cv);
     to_spinor(cv, chi);
     if constexpr (KIND == KA_WILSON) {
       const T al = (T)a.alpha;
@@ -335,7 +353,7 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   } else {
     from_spinor(acc, ov);
   }
-  store_spinor<P>(a.out, a.vs, s, ov);
+  store_spinor<P, true>(a.out, a.vs, s, ov);
 }
 
 template <int P>
