@@ -141,6 +141,22 @@ def main():
     gbs18 = BYTES_SITE[("f32", False)] * vh / (ms18 * 1e-3) / 1e9
     log(f"clover dslash f32 c18: {ms18:.4f} ms/launch, {gbs18:.1f} GB/s algorithmic")
 
+    # multi-right-hand-side clover Dslash: 12 right-hand sides share every link
+    # and clover read; algorithmic bytes 8*G + C + n*(S_in + S_out) per site
+    nr = 12
+    psis = [q.gen_spinor(g32, 100 + r, 1) for r in range(nr)]
+    outs = [g32.spinor() for _ in range(nr)]
+    D.clov_dslash_multi(outs, psis, u32[0], inv32, 1, 0)
+    q.sync()
+    timer.start()
+    for _ in range(max(1, args.steps // 10)):
+        D.clov_dslash_multi(outs, psis, u32[0], inv32, 1, 0)
+    ms_multi = timer.stop() / max(1, args.steps // 10)
+    bytes_multi = (8 * 12 + 72 + nr * 48) * 4 * vh
+    log(f"clover dslash f32 c12, {nr} rhs: {ms_multi:.3f} ms ({ms_multi / nr:.4f} ms per rhs), "
+        f"{bytes_multi / (ms_multi * 1e-3) / 1e9:.1f} GB/s algorithmic")
+    del psis, outs
+
     # copy kernel of known size (FP32 spinor, 96 B/site read + 96 B/site written)
     # for the profiler's FETCH_SIZE / WRITE_SIZE calibration; not timed
     for _ in range(5):
@@ -150,6 +166,9 @@ def main():
     extra = {"tuned": q.tune_summary().strip().splitlines(),
              "ms_per_dslash_c12": round(per_ms, 5), "ms_per_dslash_c18": round(ms18, 5),
              "gbs_c18": round(gbs18, 1),
+             "multi_rhs_12": {"ms": round(ms_multi, 4), "ms_per_rhs": round(ms_multi / nr, 5),
+                              "gbs_algorithmic": round(bytes_multi / (ms_multi * 1e-3) / 1e9, 1),
+                              "speedup_per_rhs_vs_single": round(per_ms / (ms_multi / nr), 3)},
              "gflops_c12": round(FLOPS_SITE_CLOVER * vh / (per_ms * 1e-3) / 1e9, 1)}
 
     if not args.no_solve:

@@ -186,6 +186,20 @@ int qphix_tmclov_two_flav_inverse_clover_term(qphix_dslash *d, void *const res[2
                                               const void *const fcl[2],
                                               const void *cl_offdiag, int isign);
 
+/* multi-right-hand-side forms (no reference counterpart): n right-hand sides
+ * in launches of an autotuned right-hand-side tile; links and clover are read
+ * from HBM once per tile and shared through L2 */
+int qphix_wilson_dslash_multi(qphix_dslash *d, void *const res[], const void *const psi[], int n,
+                              const void *u, int isign, int cb);
+int qphix_wilson_achimbdpsi_multi(qphix_dslash *d, void *const res[], const void *const psi[],
+                                  const void *const chi[], int n, const void *u, double alpha,
+                                  double beta, int isign, int cb);
+int qphix_clov_dslash_multi(qphix_dslash *d, void *const res[], const void *const psi[], int n,
+                            const void *u, const void *invclov, int isign, int cb);
+int qphix_clov_achimbdpsi_multi(qphix_dslash *d, void *const res[], const void *const psi[],
+                                const void *const chi[], int n, const void *u, const void *clov,
+                                double beta, int isign, int cb);
+
 /* ---- even-odd operators: EvenOddLinearOperator::operator() (linearOp.h:20-35)
  *      for EvenOddWilsonOperator (wilson.h:49-65), EvenOddCloverOperator
  *      (clover.h:162-172), EvenOddTMWilsonOperator (twisted_mass.h:62-77),
@@ -222,6 +236,10 @@ int qphix_op_nflav(const qphix_op *op);
 int qphix_op_apply(qphix_op *op, void *const res[], const void *const in[], int isign,
                    int target_cb);
 
+/* the operator on n single-flavour right-hand sides (no reference counterpart) */
+int qphix_op_apply_multi(qphix_op *op, void *const res[], const void *const in[], int n, int isign,
+                         int target_cb);
+
 /* ---- solvers: AbstractSolver::operator() (abs_solver.h:85-97) for InvCG
  *      (invcg.h:25-694), InvBiCGStab (invbicgstab.h:18-619),
  *      InvRichardsonMultiPrec (inv_richardson_multiprec.h:11-320) and
@@ -245,6 +263,12 @@ int qphix_solver_destroy(qphix_solver *s);
 int qphix_solve(qphix_solver *s, void *const x[], const void *const rhs[], double rsd_target,
                 int isign, int verbose, int target_cb, int resid_type,
                 qphix_solve_stats *st);
+/* CG on M^+M for n right-hand sides in lockstep, one launch per step for all
+ * unconverged right-hand sides (no reference counterpart; per right-hand side
+ * the iteration is that of invcg.h:88-456); st[n] */
+int qphix_solve_multi(qphix_solver *s, void *const x[], const void *const rhs[], int n,
+                      double rsd_target, int isign, int verbose, int target_cb, int resid_type,
+                      qphix_solve_stats *st);
 /* x[shift * nflav + f]; rsd_sq_final[n_shift] */
 int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[],
                        int n_shift, const double *shifts, const double *rsd_target,
@@ -268,6 +292,7 @@ int qphix_two_flav_twisted_mass(const qphix_geom *g, const double apimu[2], doub
 int qphix_axpy(const qphix_geom *g, double alpha, const void *x, void *y);             /* 214 */
 int qphix_axpby(const qphix_geom *g, double alpha, const void *x, double beta, void *y); /* 247 */
 int qphix_norm2_spinor(const qphix_geom *g, const void *x, double *n2);                 /* 278 */
+int qphix_norm2_multi(const qphix_geom *g, const void *const x[], int n, double *n2);  /* n norms, one launch */
 int qphix_axpy_norm2(const qphix_geom *g, double alpha, const void *x, void *y, double *n2); /* 312 */
 int qphix_xmy_norm2(const qphix_geom *g, void *res, const void *x, const void *y, double *n2); /* 345: res = x - y */
 int qphix_xmy2_norm2(const qphix_geom *g, const void *x, void *y, double *n2);        /* 360: y = y - x */

@@ -329,6 +329,17 @@ class Dslash:
         call("qphix_tmclov_two_flav_achimbdpsi", self.h, _pa(res), _pa(psi), _pa(chi), u.p,
              clov.p, C.c_double(beta), C.c_double(eps), isign, cb)
 
+    def dslash_multi(self, res, psi, u, isign, cb):
+        call("qphix_wilson_dslash_multi", self.h, _pa(res), _pa(psi), len(psi), u.p, isign, cb)
+
+    def clov_dslash_multi(self, res, psi, u, invclov, isign, cb):
+        call("qphix_clov_dslash_multi", self.h, _pa(res), _pa(psi), len(psi), u.p, invclov.p,
+             isign, cb)
+
+    def clov_achimbdpsi_multi(self, res, psi, chi, u, clov, beta, isign, cb):
+        call("qphix_clov_achimbdpsi_multi", self.h, _pa(res), _pa(psi), _pa(chi), len(psi), u.p,
+             clov.p, C.c_double(beta), isign, cb)
+
     def two_flav_inverse_clover_term(self, res, psi, fcl, cl_offdiag, isign):
         call("qphix_tmclov_two_flav_inverse_clover_term", self.h, _pa(res), _pa(psi), _pa(fcl),
              cl_offdiag.p, isign)
@@ -374,6 +385,9 @@ class Operator:
         inp = inp if isinstance(inp, (list, tuple)) else [inp]
         call("qphix_op_apply", self.h, _pa(res), _pa(inp), isign, target_cb)
 
+    def apply_multi(self, res, inp, isign, target_cb=1):
+        call("qphix_op_apply_multi", self.h, _pa(res), _pa(inp), len(inp), isign, target_cb)
+
 
 class Solver:
     def __init__(self, h, M, keep=()):
@@ -393,6 +407,14 @@ class Solver:
         call("qphix_solve", self.h, _pa(x), _pa(rhs), C.c_double(rsd_target), isign,
              int(verbose), target_cb, resid, C.byref(st))
         return st
+
+    def solve_multi(self, x, rhs, rsd_target, isign=1, target_cb=1, resid=RESID_RELATIVE,
+                    verbose=False):
+        n = len(rhs)
+        st = (SolveStats * n)()
+        call("qphix_solve_multi", self.h, _pa(x), _pa(rhs), n, C.c_double(rsd_target), isign,
+             int(verbose), target_cb, resid, st)
+        return list(st)
 
 
 def InvCG(M, max_iters):
@@ -447,6 +469,12 @@ def norm2(geom, x):
     r = C.c_double()
     call("qphix_norm2_spinor", geom.h, x.p, C.byref(r))
     return r.value
+
+
+def norm2_multi(geom, xs):
+    o = (C.c_double * len(xs))()
+    call("qphix_norm2_multi", geom.h, _pa(xs), len(xs), o)
+    return list(o)
 
 
 def inner_product(geom, l, r):

@@ -42,6 +42,7 @@ struct qphix_op {
   int nflav;
   void *tmp[2];
   void *tmp2[2];
+  std::vector<void *> mtmp;  // per-right-hand-side temporaries of qphix_op_apply_multi
 };
 enum { S_CG = 0, S_BICGSTAB = 1, S_RICHARDSON = 2, S_MINVCG = 3 };
 struct qphix_solver {
@@ -55,6 +56,8 @@ struct qphix_solver {
   int mdagm;
   // MInvCG
   int max_shifts;
+  // multi-right-hand-side CG work spinors (grown on demand)
+  std::vector<void *> mwork;
 };
 
 // ---------------------------------------------------------------------------
@@ -66,8 +69,9 @@ static int g_device = 0;
 static hipStream_t g_own_stream = nullptr;
 static hipStream_t g_stream = nullptr;
 static double *g_partials = nullptr;  // device, kMaxBlasBlocks * 3
-static double *g_dres = nullptr;      // device, 4
-static double *g_hres = nullptr;      // pinned host, 4
+static double *g_dres = nullptr;      // device, kResults
+static double *g_hres = nullptr;      // pinned host, kResults
+static constexpr int kResults = 4 * kMaxRhs;
 static hipEvent_t g_ev0, g_ev1;
 static int g_hop_block = 256;
 
@@ -127,8 +131,8 @@ extern "C" int qphix_init(int device) {
   HIPCHK(hipStreamCreate(&g_own_stream));
   g_stream = g_own_stream;
   HIPCHK(hipMalloc(&g_partials, sizeof(double) * kMaxBlasBlocks * 3));
-  HIPCHK(hipMalloc(&g_dres, sizeof(double) * 4));
-  HIPCHK(hipHostMalloc(&g_hres, sizeof(double) * 4, hipHostMallocDefault));
+  HIPCHK(hipMalloc(&g_dres, sizeof(double) * kResults));
+  HIPCHK(hipHostMalloc(&g_hres, sizeof(double) * kResults, hipHostMallocDefault));
   HIPCHK(hipEventCreate(&g_ev0));
   HIPCHK(hipEventCreate(&g_ev1));
   g_init = true;
@@ -570,10 +574,11 @@ extern "C" int qphix_dslash_destroy(qphix_dslash *d) {
 struct HopCfg {
   int block, bz, slabs;
   double ms;
+  int rt;  // right-hand sides per launch (multi-RHS keys)
 };
 static std::map<std::string, HopCfg> g_tuned;
-static int g_tune_enabled = -1;         // -1: read QPHIX_TUNE on first use
-static HopCfg g_forced = {0, 0, 0, 0};  // block > 0: every hop uses this configuration
+static int g_tune_enabled = -1;            // -1: read QPHIX_TUNE on first use
+static HopCfg g_forced = {0, 0, 0, 0, 0};  // block > 0: every hop uses this configuration
 
 static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block) {
   switch (g.prec) {
@@ -583,79 +588,113 @@ static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const H
   }
 }
 
-static int tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, HopCfg &best) {
-  best.block = g_hop_block;
-  best.bz = 0;
-  best.slabs = 1;
-  best.ms = -1;
+static bool tuning_on(const GeomDesc &g) {
+  if (g_tune_enabled < 0) {
+    const char *e = getenv("QPHIX_TUNE");
+    g_tune_enabled = (e && e[0] == '0') ? 0 : 1;
+  }
+  // small lattices: the traversal order does not matter and tuning would dominate
+  return g_tune_enabled && g.vh >= (1L << 17);
+}
+
+static size_t spinor_bytes(const GeomDesc &g) {
+  return g.prec == P64 ? field_bytes<P64>(kSpinorReals, g.vs)
+         : g.prec == P32 ? field_bytes<P32>(kSpinorReals, g.vs) : field_bytes<P16>(kSpinorReals, g.vs);
+}
+
+// time `reps` launches of a; returns ms per launch (1e30 if it cannot launch)
+static double time_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block, int reps) {
+  if (launch_hop_raw(g, kind, isign, a, block) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1e30;
+  }
+  if (hipEventRecord(g_ev0, g_stream) != hipSuccess) return 1e30;
+  for (int r = 0; r < reps; r++)
+    if (launch_hop_raw(g, kind, isign, a, block) != hipSuccess) return 1e30;
+  if (hipEventRecord(g_ev1, g_stream) != hipSuccess || hipEventSynchronize(g_ev1) != hipSuccess) return 1e30;
+  float f = 0;
+  if (hipEventElapsedTime(&f, g_ev0, g_ev1) != hipSuccess) return 1e30;
+  return f / reps;
+}
+
+static void tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, HopCfg &best) {
+  best = HopCfg{g_hop_block, 0, 1, -1, 1};
   if (g_forced.block > 0) {
     best = g_forced;
     if (best.bz > 0 && (g.L[2] % best.bz || g.L[3] % best.slabs)) {
       best.bz = 0;
       best.slabs = 1;
     }
-    return QPHIX_OK;
+    return;
   }
-  if (g_tune_enabled < 0) {
-    const char *e = getenv("QPHIX_TUNE");
-    g_tune_enabled = (e && e[0] == '0') ? 0 : 1;
-  }
-  // small lattices: the traversal order does not matter and tuning would dominate
-  if (!g_tune_enabled || g.vh < (1L << 17)) return QPHIX_OK;
+  if (!tuning_on(g)) return;
   std::vector<HopCfg> cand;
   for (int blk : {128, 256}) {
-    cand.push_back({blk, 0, 1, 0});
+    cand.push_back({blk, 0, 1, 0, 1});
     for (int bz : {2, 4, 6, 8, 12, 16, 24})
       for (int sl : {8, 16})
-        if (bz <= g.L[2] && g.L[2] % bz == 0 && g.L[3] % sl == 0 && g.L[3] / sl >= 2) cand.push_back({blk, bz, sl, 0});
+        if (bz <= g.L[2] && g.L[2] % bz == 0 && g.L[3] % sl == 0 && g.L[3] / sl >= 2)
+          cand.push_back({blk, bz, sl, 0, 1});
   }
   DevBuf scratch;
-  size_t bytes = g.prec == P64 ? field_bytes<P64>(kSpinorReals, g.vs)
-                 : g.prec == P32 ? field_bytes<P32>(kSpinorReals, g.vs) : field_bytes<P16>(kSpinorReals, g.vs);
-  if (hipMalloc(&scratch.p, bytes) != hipSuccess) {
+  if (hipMalloc(&scratch.p, spinor_bytes(g)) != hipSuccess) {
     (void)hipGetLastError();
-    return QPHIX_OK;  // keep the default configuration
+    return;  // keep the default configuration
   }
   HopArgs a = a0;
+  a.nrhs = 1;
   a.out = scratch.p;
   for (HopCfg &c : cand) {
     a.bz = c.bz;
     a.slabs = c.slabs;
-    if (launch_hop_raw(g, kind, isign, a, c.block) != hipSuccess) {
-      (void)hipGetLastError();
-      c.ms = 1e30;
-      continue;
-    }
-    HIPCHK(hipEventRecord(g_ev0, g_stream));
-    for (int r = 0; r < 3; r++) HIPCHK(launch_hop_raw(g, kind, isign, a, c.block));
-    HIPCHK(hipEventRecord(g_ev1, g_stream));
-    HIPCHK(hipEventSynchronize(g_ev1));
-    float f = 0;
-    HIPCHK(hipEventElapsedTime(&f, g_ev0, g_ev1));
-    c.ms = f / 3;
+    c.ms = time_hop(g, kind, isign, a, c.block, 3);
     if (best.ms < 0 || c.ms < best.ms) best = c;
   }
-  return QPHIX_OK;
 }
 
-static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *res, const void *psi, const void *u,
-                      const void *clov, const void *chi, double coef, double alpha, double mu, double muinv,
-                      double rho) {
-  ENSURE_INIT();
-  CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(u);
-  if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1, got %d", isign);
-  TRY(check_cb(cb));
-  if ((kind == KD_CLOVER || kind == KA_CLOVER || kind == KD_FULL) && !clov)
-    return fail(QPHIX_EINVAL, "clover field required");
-  if ((kind == KA_WILSON || kind == KA_CLOVER || kind == KA_TM) && !chi) return fail(QPHIX_EINVAL, "chi required");
+// right-hand-side tile for n right-hand sides, given the single-RHS traversal
+static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs &a0, int n, HopCfg &cfg) {
+  cfg.rt = n < kMaxRhs ? n : kMaxRhs;
+  if (g_forced.block > 0 || !tuning_on(g) || n == 1) return;
+  const int rmax = cfg.rt;
+  std::vector<DevBuf> outs(rmax);
+  for (int r = 0; r < rmax; r++)
+    if (hipMalloc(&outs[r].p, spinor_bytes(g)) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+  HopArgs a = a0;
+  a.bz = cfg.bz;
+  a.slabs = cfg.slabs;
+  double best = -1;
+  for (int rt : {1, 2, 4, 8, 16, 32}) {
+    if (rt > rmax) break;
+    a.nrhs = rt;
+    for (int r = 0; r < rt; r++) {
+      a.psis[r] = a0.psis[r % n];
+      a.chis[r] = a0.chis[r % n];
+      a.outs[r] = outs[r].p;
+    }
+    if (rt == 1) {
+      a.psi = a.psis[0];
+      a.chi = a.chis[0];
+      a.out = a.outs[0];
+    }
+    const double per = time_hop(g, kind, isign, a, cfg.block, 2) / rt;
+    if (best < 0 || per < best) {
+      best = per;
+      cfg.rt = rt;
+    }
+  }
+}
+
+static HopArgs make_hop_args(const qphix_dslash *d, int cb, const void *u, const void *clov, double coef,
+                             double alpha, double mu, double muinv, double rho) {
   const GeomDesc &g = d->g->d;
   HopArgs a;
   memset(&a, 0, sizeof(a));
-  a.psi = psi;
-  a.out = res;
   a.u = u;
   a.clov = clov;
-  a.chi = chi;
   a.vs = g.vs;
   a.vh = (int)g.vh;
   a.nxh = g.L[0] / 2;
@@ -676,22 +715,88 @@ static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *
   a.mu = mu;
   a.muinv = muinv;
   a.rho = rho;
-  a.bz = 0;
   a.slabs = 1;
+  a.nrhs = 1;
+  return a;
+}
+
+// one kernel-family member applied to n right-hand sides (n = 1: the plain call)
+static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void *const *res,
+                        const void *const *psi, const void *const *chi, int n, const void *u, const void *clov,
+                        double coef, double alpha, double mu, double muinv, double rho) {
+  ENSURE_INIT();
+  CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(u);
+  if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1, got %d", isign);
+  TRY(check_cb(cb));
+  if (n < 1) return fail(QPHIX_EDOMAIN, "number of right-hand sides %d", n);
+  if ((kind == KD_CLOVER || kind == KA_CLOVER || kind == KD_FULL) && !clov)
+    return fail(QPHIX_EINVAL, "clover field required");
+  const bool needs_chi = kind == KA_WILSON || kind == KA_CLOVER || kind == KA_TM;
+  for (int r = 0; r < n; r++) {
+    if (!res[r] || !psi[r]) return fail(QPHIX_EINVAL, "null spinor for right-hand side %d", r);
+    if (needs_chi && (!chi || !chi[r])) return fail(QPHIX_EINVAL, "chi required");
+  }
+  const GeomDesc &g = d->g->d;
+  HopArgs a = make_hop_args(d, cb, u, clov, coef, alpha, mu, muinv, rho);
   char key[128];
   snprintf(key, sizeof(key), "p%d c%d k%d s%d %dx%dx%dx%d", g.prec, g.c12, kind, isign, g.L[0], g.L[1], g.L[2],
            g.L[3]);
   auto it = g_tuned.find(key);
   if (it == g_tuned.end()) {
     HopCfg c;
-    TRY(tune_hop(g, kind, isign, a, c));
+    HopArgs t = a;
+    t.psi = psi[0];
+    t.chi = needs_chi ? chi[0] : nullptr;
+    tune_hop(g, kind, isign, t, c);
     it = g_tuned.emplace(key, c).first;
   }
-  a.bz = it->second.bz;
-  a.slabs = it->second.slabs;
-  hipError_t e = launch_hop_raw(g, kind, isign, a, it->second.block);
-  if (e != hipSuccess) return fail(QPHIX_EHIP, "hop kernel launch: %s", hipGetErrorString(e));
+  HopCfg cfg = it->second;
+  if (n > 1) {
+    char mkey[160];
+    snprintf(mkey, sizeof(mkey), "%s n%d", key, n);
+    auto mt = g_tuned.find(mkey);
+    if (mt == g_tuned.end()) {
+      HopCfg c = cfg;
+      HopArgs t = a;
+      for (int r = 0; r < n && r < kMaxRhs; r++) {
+        t.psis[r] = psi[r];
+        t.chis[r] = needs_chi ? chi[r] : nullptr;
+      }
+      tune_rhs_tile(g, kind, isign, t, n, c);
+      mt = g_tuned.emplace(mkey, c).first;
+    }
+    cfg = mt->second;
+  }
+  a.bz = cfg.bz;
+  a.slabs = cfg.slabs;
+  const int rt = n == 1 ? 1 : (cfg.rt > 0 ? cfg.rt : 1);
+  for (int r0 = 0; r0 < n; r0 += rt) {
+    const int m = (n - r0) < rt ? (n - r0) : rt;
+    a.nrhs = m;
+    if (m == 1) {
+      a.psi = psi[r0];
+      a.out = res[r0];
+      a.chi = needs_chi ? chi[r0] : nullptr;
+    } else {
+      for (int r = 0; r < m; r++) {
+        a.psis[r] = psi[r0 + r];
+        a.outs[r] = res[r0 + r];
+        a.chis[r] = needs_chi ? chi[r0 + r] : nullptr;
+      }
+    }
+    hipError_t e = launch_hop_raw(g, kind, isign, a, cfg.block);
+    if (e != hipSuccess) return fail(QPHIX_EHIP, "hop kernel launch: %s", hipGetErrorString(e));
+  }
   return QPHIX_OK;
+}
+
+static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *res, const void *psi, const void *u,
+                      const void *clov, const void *chi, double coef, double alpha, double mu, double muinv,
+                      double rho) {
+  void *r[1] = {res};
+  const void *p[1] = {psi};
+  const void *c[1] = {chi};
+  return launch_hop_n(d, kind, isign, cb, r, p, c, 1, u, clov, coef, alpha, mu, muinv, rho);
 }
 
 // mode 0: no tuning (natural order, default block); 1: autotune on first use;
@@ -702,7 +807,7 @@ extern "C" int qphix_tune_config(int mode, int block, int bz, int slabs) {
     return fail(QPHIX_EDOMAIN, "forced configuration block %d bz %d slabs %d", block, bz, slabs);
   g_tuned.clear();
   g_tune_enabled = mode == 1 ? 1 : 0;
-  g_forced = mode == 2 ? HopCfg{block, bz, bz > 0 ? slabs : 1, 0} : HopCfg{0, 0, 0, 0};
+  g_forced = mode == 2 ? HopCfg{block, bz, bz > 0 ? slabs : 1, 0, kMaxRhs} : HopCfg{0, 0, 0, 0, 0};
   return QPHIX_OK;
 }
 
@@ -712,8 +817,8 @@ extern "C" const char *qphix_tune_summary(void) {
   out.clear();
   for (auto &kv : g_tuned) {
     char line[256];
-    snprintf(line, sizeof(line), "%s: block %d bz %d slabs %d %.4f ms\n", kv.first.c_str(), kv.second.block,
-             kv.second.bz, kv.second.slabs, kv.second.ms);
+    snprintf(line, sizeof(line), "%s: block %d bz %d slabs %d rhs-tile %d %.4f ms\n", kv.first.c_str(),
+             kv.second.block, kv.second.bz, kv.second.slabs, kv.second.rt, kv.second.ms);
     out += line;
   }
   return out.c_str();
@@ -734,6 +839,27 @@ extern "C" int qphix_clov_achimbdpsi(qphix_dslash *d, void *res, const void *psi
                                      const void *clov, double beta, int isign, int cb) {
   CHKPTR(d);
   return launch_hop(d, KA_CLOVER, isign, cb, res, psi, u, clov, chi, -beta, 0, 0, 0, d->p.prec_mass_rho);
+}
+// multi-right-hand-side forms (no reference counterpart): one launch per
+// right-hand-side tile, links and clover shared through L2
+extern "C" int qphix_wilson_dslash_multi(qphix_dslash *d, void *const res[], const void *const psi[], int n,
+                                         const void *u, int isign, int cb) {
+  return launch_hop_n(d, KD_WILSON, isign, cb, res, psi, nullptr, n, u, nullptr, 1.0, 0, 0, 0, 0);
+}
+extern "C" int qphix_wilson_achimbdpsi_multi(qphix_dslash *d, void *const res[], const void *const psi[],
+                                             const void *const chi[], int n, const void *u, double alpha,
+                                             double beta, int isign, int cb) {
+  return launch_hop_n(d, KA_WILSON, isign, cb, res, psi, chi, n, u, nullptr, -beta, alpha, 0, 0, 0);
+}
+extern "C" int qphix_clov_dslash_multi(qphix_dslash *d, void *const res[], const void *const psi[], int n,
+                                       const void *u, const void *invclov, int isign, int cb) {
+  return launch_hop_n(d, KD_CLOVER, isign, cb, res, psi, nullptr, n, u, invclov, 1.0, 0, 0, 0, 0);
+}
+extern "C" int qphix_clov_achimbdpsi_multi(qphix_dslash *d, void *const res[], const void *const psi[],
+                                           const void *const chi[], int n, const void *u, const void *clov,
+                                           double beta, int isign, int cb) {
+  CHKPTR(d);
+  return launch_hop_n(d, KA_CLOVER, isign, cb, res, psi, chi, n, u, clov, -beta, 0, 0, 0, d->p.prec_mass_rho);
 }
 extern "C" int qphix_clov_mult(qphix_dslash *d, void *res, const void *chi, const void *clov) {
   ENSURE_INIT();
@@ -810,8 +936,7 @@ extern "C" int qphix_op_create(const qphix_geom *g, const qphix_op_params *p, qp
   CHKPTR(g); CHKPTR(p); CHKPTR(out);
   if (p->kind < QPHIX_OP_WILSON || p->kind > QPHIX_OP_NDTMCLOVER)
     return fail(QPHIX_EDOMAIN, "unknown operator kind %d", p->kind);
-  qphix_op *op = new qphix_op;
-  memset(op, 0, sizeof(*op));
+  qphix_op *op = new qphix_op();
   op->g = g;
   op->p = *p;
   op->d.g = g;
@@ -837,6 +962,7 @@ extern "C" int qphix_op_destroy(qphix_op *op) {
     qphix_free(op->tmp[f]);
     qphix_free(op->tmp2[f]);
   }
+  for (void *t : op->mtmp) qphix_free(t);
   delete op;
   return QPHIX_OK;
 }
@@ -878,6 +1004,76 @@ extern "C" int qphix_op_apply(qphix_op *op, void *const res[], const void *const
     }
   }
   return fail(QPHIX_EDOMAIN, "unknown operator kind");
+}
+
+// the operator on n single-flavour right-hand sides, two multi-RHS launches
+extern "C" int qphix_op_apply_multi(qphix_op *op, void *const res[], const void *const in[], int n, int isign,
+                                    int t) {
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  TRY(check_cb(t));
+  if (op->nflav != 1) return fail(QPHIX_EDOMAIN, "multi-RHS apply is for single-flavour operators");
+  if (n < 1) return fail(QPHIX_EDOMAIN, "number of right-hand sides %d", n);
+  while ((int)op->mtmp.size() < n) {
+    void *p = nullptr;
+    TRY(qphix_alloc_spinor(op->g, &p));
+    op->mtmp.push_back(p);
+  }
+  const int o = 1 - t;
+  const qphix_op_params &p = op->p;
+  qphix_dslash *D = &op->d;
+  void *const *tmp = op->mtmp.data();
+  const void *const *ctmp = (const void *const *)op->mtmp.data();
+  switch (p.kind) {
+    case QPHIX_OP_WILSON: {
+      const double al = 4.0 + p.mass;
+      TRY(launch_hop_n(D, KD_WILSON, isign, o, tmp, in, nullptr, n, p.u[o], nullptr, 1.0, 0, 0, 0, 0));
+      return launch_hop_n(D, KA_WILSON, isign, t, res, ctmp, in, n, p.u[t], nullptr, -0.25 / al, al, 0, 0, 0);
+    }
+    case QPHIX_OP_CLOVER:
+      TRY(launch_hop_n(D, KD_CLOVER, isign, o, tmp, in, nullptr, n, p.u[o], p.invclov[o], 1.0, 0, 0, 0, 0));
+      return launch_hop_n(D, KA_CLOVER, isign, t, res, ctmp, in, n, p.u[t], p.clov[t], -0.25, 0, 0, 0,
+                          D->p.prec_mass_rho);
+    case QPHIX_OP_TMWILSON: {
+      const double al = 4.0 + p.mass, mu = p.twisted_mass;
+      TRY(launch_hop_n(D, KD_TM, isign, o, tmp, in, nullptr, n, p.u[o], nullptr, 1.0, 0, mu / al,
+                       al / (mu * mu + al * al), 0));
+      return launch_hop_n(D, KA_TM, isign, t, res, ctmp, in, n, p.u[t], nullptr, -0.25, al, mu / al, 0, 0);
+    }
+    case QPHIX_OP_TMCLOVER: {
+      const int clidx = isign == 1 ? 0 : 1;
+      TRY(launch_hop_n(D, KD_FULL, isign, o, tmp, in, nullptr, n, p.u[o], p.fullclov[o][clidx], 1.0, 0, 0, 0, 0));
+      return launch_hop_n(D, KA_CLOVER, isign, t, res, ctmp, in, n, p.u[t], p.clov[t], -0.25, 0, 0, 0,
+                          D->p.prec_mass_rho);
+    }
+  }
+  return fail(QPHIX_EDOMAIN, "unknown operator kind");
+}
+
+// n right-hand sides of one BLAS operation in one launch; out gets n * nred values
+static int run_blas_multi(const qphix_geom *g, int op, const MultiBlasArgs &m, double *out) {
+  HIPCHK(blas_multi_launch(g->d, op, m, g_partials, g_dres, g_stream));
+  const int nr = blas_nred(op);
+  if (nr > 0) {
+    HIPCHK(hipMemcpyAsync(g_hres, g_dres, (size_t)m.n * nr * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (out)
+      for (int k = 0; k < m.n * nr; k++) out[k] = g_hres[k];
+  }
+  return QPHIX_OK;
+}
+static MultiBlasArgs margs(int n) {
+  MultiBlasArgs m;
+  memset(&m, 0, sizeof(m));
+  m.n = n;
+  return m;
+}
+extern "C" int qphix_norm2_multi(const qphix_geom *g, const void *const x[], int n, double *n2) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(x); CHKPTR(n2);
+  if (n < 1 || n > kMaxRhs) return fail(QPHIX_EDOMAIN, "n = %d right-hand sides (max %d)", n, kMaxRhs);
+  MultiBlasArgs m = margs(n);
+  for (int r = 0; r < n; r++) m.x[r] = x[r];
+  return run_blas_multi(g, B_NORM2, m, n2);
 }
 
 // ---------------------------------------------------------------------------
@@ -933,7 +1129,7 @@ static void cmul(double *o, const double *a, const double *b) {
 
 extern "C" int qphix_cg_create(qphix_op *M, int max_iters, qphix_solver **s) {
   CHKPTR(M); CHKPTR(s);
-  qphix_solver *x = new qphix_solver;
+  qphix_solver *x = new qphix_solver();
   x->kind = S_CG;
   x->M = M;
   x->max_iters = max_iters;
@@ -969,6 +1165,7 @@ extern "C" int qphix_minvcg_create(qphix_op *M, int max_iters, int max_shifts, q
 extern "C" int qphix_solver_destroy(qphix_solver *s) {
   if (!s) return QPHIX_OK;
   for (void *p : s->work) qphix_free(p);
+  for (void *p : s->mwork) qphix_free(p);
   delete s;
   return QPHIX_OK;
 }
@@ -1258,6 +1455,144 @@ extern "C" int qphix_solve(qphix_solver *s, void *const x[], const void *const r
     case S_RICHARDSON: return solve_richardson(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
   }
   return fail(QPHIX_EINVAL, "qphix_solve on a multishift solver; use qphix_minvcg_solve");
+}
+
+// CG on M^+M for n right-hand sides advanced in lockstep (invcg.h:88-456 per
+// right-hand side): every operator application and BLAS sweep is one launch for
+// all still-unconverged right-hand sides; converged ones drop out.
+static int solve_cg_multi(qphix_solver *s, void *const *x, const void *const *rhs, int n, double rsd_target,
+                          int isign, int verbose, int cb, int rtype, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  while ((int)s->mwork.size() < 4 * n) {
+    void *p = nullptr;
+    TRY(qphix_alloc_spinor(g, &p));
+    s->mwork.push_back(p);
+  }
+  void **mp = &s->mwork[0], **mmp = &s->mwork[n], **pp = &s->mwork[2 * n], **rr = &s->mwork[3 * n];
+  std::vector<double> chi_sq(n), rsd_sq(n), cp(n), c(n), d(n), tmp(n);
+  std::vector<int> act;
+  for (int r = 0; r < n; r++) {
+    memset(&st[r], 0, sizeof(st[r]));
+  }
+  auto sub = [&](void *const *v, const std::vector<int> &ix, std::vector<void *> &o) {
+    o.clear();
+    for (int i : ix) o.push_back(v[i]);
+  };
+  std::vector<int> all(n);
+  for (int r = 0; r < n; r++) all[r] = r;
+  // chi_sq, initial residual
+  {
+    MultiBlasArgs m = margs(n);
+    for (int r = 0; r < n; r++) m.x[r] = rhs[r];
+    TRY(run_blas_multi(g, B_NORM2, m, chi_sq.data()));
+  }
+  for (int r = 0; r < n; r++) rsd_sq[r] = rsd_target * rsd_target * (rtype == QPHIX_RESID_RELATIVE ? chi_sq[r] : 1.0);
+  auto mdagm = [&](const std::vector<int> &ix, void *const *in) -> int {
+    std::vector<void *> vi, vm, vmm;
+    sub(in, ix, vi);
+    sub(mp, ix, vm);
+    sub(mmp, ix, vmm);
+    TRY(qphix_op_apply_multi(M, vm.data(), (const void *const *)vi.data(), (int)ix.size(), isign, cb));
+    TRY(qphix_op_apply_multi(M, vmm.data(), (const void *const *)vm.data(), (int)ix.size(), -isign, cb));
+    for (int i : ix) st[i].mv_apps += 2;
+    return QPHIX_OK;
+  };
+  auto true_resid = [&](const std::vector<int> &ix) -> int {  // rr = rhs - M^+M x, cp
+    TRY(mdagm(ix, x));
+    MultiBlasArgs m = margs((int)ix.size());
+    for (size_t k = 0; k < ix.size(); k++) {
+      m.o0[k] = rr[ix[k]];
+      m.x[k] = rhs[ix[k]];
+      m.y[k] = mmp[ix[k]];
+    }
+    TRY(run_blas_multi(g, B_XMYNORM2, m, tmp.data()));
+    for (size_t k = 0; k < ix.size(); k++) cp[ix[k]] = tmp[k];
+    return QPHIX_OK;
+  };
+  TRY(true_resid(all));
+  for (int r = 0; r < n; r++) {
+    if (cp[r] <= rsd_sq[r]) {
+      st[r].n_iters = 0;
+      st[r].rsd_sq_final = cp[r] / chi_sq[r];
+    } else {
+      act.push_back(r);
+    }
+  }
+  {
+    MultiBlasArgs m = margs((int)act.size());
+    for (size_t k = 0; k < act.size(); k++) {
+      m.o0[k] = pp[act[k]];
+      m.x[k] = rr[act[k]];
+    }
+    if (!act.empty()) TRY(run_blas_multi(g, B_COPY, m, nullptr));
+  }
+  std::vector<int> done;
+  for (int k = 0; k <= s->max_iters && !act.empty(); ++k) {
+    for (int i : act) c[i] = cp[i];
+    TRY(mdagm(act, pp));
+    MultiBlasArgs m = margs((int)act.size());
+    for (size_t q = 0; q < act.size(); q++) m.x[q] = mp[act[q]];
+    TRY(run_blas_multi(g, B_NORM2, m, tmp.data()));
+    m = margs((int)act.size());
+    for (size_t q = 0; q < act.size(); q++) {
+      const int i = act[q];
+      m.o0[q] = rr[i];
+      m.x[q] = mmp[i];
+      m.o1[q] = x[i];
+      m.y[q] = pp[i];
+      m.a[q] = c[i] / tmp[q];
+    }
+    TRY(run_blas_multi(g, B_RMAMMP, m, tmp.data()));
+    std::vector<int> still;
+    for (size_t q = 0; q < act.size(); q++) {
+      const int i = act[q];
+      cp[i] = tmp[q];
+      if (verbose) printf("CG[%d]: iter %d: r = %e target = %e\n", i, k, cp[i], rsd_sq[i]);
+      if (cp[i] < rsd_sq[i]) {
+        st[i].n_iters = k;
+        done.push_back(i);
+      } else {
+        still.push_back(i);
+      }
+    }
+    act = still;
+    if (act.empty()) break;
+    m = margs((int)act.size());
+    for (size_t q = 0; q < act.size(); q++) {
+      const int i = act[q];
+      m.o0[q] = pp[i];
+      m.x[q] = rr[i];
+      m.a[q] = cp[i] / c[i];
+    }
+    TRY(run_blas_multi(g, B_AYPX, m, nullptr));
+  }
+  for (int i : act) {
+    st[i].n_iters = s->max_iters;
+    st[i].rsd_sq_final = cp[i];
+    st[i].status = QPHIX_ENOCONV;
+  }
+  if (!done.empty()) {
+    TRY(true_resid(done));
+    for (int i : done) st[i].rsd_sq_final = cp[i];
+  }
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *const rhs[], int n, double rsd_target,
+                                 int isign, int verbose, int target_cb, int resid_type, qphix_solve_stats *st) {
+  ENSURE_INIT();
+  CHKPTR(s); CHKPTR(x); CHKPTR(rhs); CHKPTR(st);
+  TRY(check_cb(target_cb));
+  if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1");
+  if (s->kind != S_CG) return fail(QPHIX_EINVAL, "multi-right-hand-side solves use a CG solver");
+  if (s->M->nflav != 1) return fail(QPHIX_EDOMAIN, "multi-right-hand-side CG is for single-flavour operators");
+  if (n < 1) return fail(QPHIX_EDOMAIN, "number of right-hand sides %d", n);
+  for (int r0 = 0; r0 < n; r0 += kMaxRhs) {
+    const int m = (n - r0) < kMaxRhs ? (n - r0) : kMaxRhs;
+    TRY(solve_cg_multi(s, x + r0, rhs + r0, m, rsd_target, isign, verbose, target_cb, resid_type, st + r0));
+  }
+  return QPHIX_OK;
 }
 
 // minvcg.h:77-387

@@ -245,6 +245,7 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
 }
 
 template <int NR> __device__ inline void block_reduce_store(double *red, double *partials) {
+  // partials: this launch's (or this right-hand side's) partial array
   __shared__ double lds[kBlasThreads / 64][NR > 0 ? NR : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -374,6 +375,92 @@ hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st)
     case P32: k_mshift<P32><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
     case P16: k_mshift<P16><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
     default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int P, int OP>
+__global__ void __launch_bounds__(kBlasThreads) k_blas_multi(const MultiBlasArgs m, size_t vs, long vh,
+                                                              double *partials) {
+  constexpr int NR = nred_of(OP);
+  const int r = blockIdx.y;
+  BlasArgs a;
+  a.x = m.x[r];
+  a.y = m.y[r];
+  a.z = nullptr;
+  a.w = nullptr;
+  a.o0 = m.o0[r];
+  a.o1 = m.o1[r];
+  a.o2 = nullptr;
+  a.o3 = nullptr;
+  a.a[0] = m.a[r];
+  for (int k = 1; k < 6; k++) a.a[k] = 0;
+  double red[NR > 0 ? NR : 1] = {0};
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < vh; s += stride) blas_site<P, OP>(a, vs, s, red);
+  if constexpr (NR > 0) block_reduce_store<NR>(red, partials + (size_t)r * gridDim.x * NR);
+}
+
+template <int NR>
+__global__ void __launch_bounds__(kBlasThreads) k_finalize_multi(const double *partials, int nblocks, double *out) {
+  const int r = blockIdx.x;
+  const double *p = partials + (size_t)r * nblocks * NR;
+  double red[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) red[k] = 0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NR; k++) red[k] += p[b * NR + k];
+  __shared__ double lds[kBlasThreads / 64][NR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double v = red[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) lds[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double v = 0;
+    for (int w = 0; w < kBlasThreads / 64; w++) v += lds[w][threadIdx.x];
+    out[r * NR + threadIdx.x] = v;
+  }
+}
+
+template <int P>
+static hipError_t multi_prec(const GeomDesc &g, int op, const MultiBlasArgs &m, double *partials, int nb,
+                             hipStream_t st) {
+  const dim3 grid(nb, m.n);
+#define MOP(o) \
+  case o: k_blas_multi<P, o><<<grid, kBlasThreads, 0, st>>>(m, g.vs, g.vh, partials); break;
+  switch (op) {
+    MOP(B_COPY) MOP(B_ZERO) MOP(B_NORM2) MOP(B_XMYNORM2) MOP(B_RMAMMP) MOP(B_AYPX) MOP(B_AXPY) MOP(B_AXPYNORM2)
+    default: return hipErrorInvalidValue;
+  }
+#undef MOP
+  return hipGetLastError();
+}
+
+hipError_t blas_multi_launch(const GeomDesc &g, int op, const MultiBlasArgs &m, double *partials, double *out,
+                             hipStream_t st) {
+  if (m.n < 1 || m.n > kMaxRhs) return hipErrorInvalidValue;
+  long nb = (g.vh + kBlasThreads - 1) / kBlasThreads;
+  if (nb > kMaxBlasBlocks / m.n) nb = kMaxBlasBlocks / m.n;
+  if (nb < 1) nb = 1;
+  hipError_t e;
+  switch (g.prec) {
+    case P64: e = multi_prec<P64>(g, op, m, partials, (int)nb, st); break;
+    case P32: e = multi_prec<P32>(g, op, m, partials, (int)nb, st); break;
+    default: e = multi_prec<P16>(g, op, m, partials, (int)nb, st); break;
+  }
+  if (e != hipSuccess) return e;
+  const int nr = nred_of(op);
+  if (nr == 0) return hipSuccess;
+  switch (nr) {
+    case 1: k_finalize_multi<1><<<m.n, kBlasThreads, 0, st>>>(partials, (int)nb, out); break;
+    case 2: k_finalize_multi<2><<<m.n, kBlasThreads, 0, st>>>(partials, (int)nb, out); break;
+    case 3: k_finalize_multi<3><<<m.n, kBlasThreads, 0, st>>>(partials, (int)nb, out); break;
   }
   return hipGetLastError();
 }

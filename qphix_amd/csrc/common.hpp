@@ -52,6 +52,7 @@ template <> struct PT<P16> {
   static constexpr bool scaled = true;
 };
 
+constexpr int kMaxRhs = 32;  // right-hand sides per multi-RHS launch
 constexpr int kSpinorReals = 24;
 constexpr int kCloverReals = 72;
 constexpr int kFullCloverReals = 144;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+
+#include "common.hpp"
 #include <stdint.h>
 
 namespace qphix_amd {
@@ -67,5 +69,19 @@ struct MShiftArgs {
   int do_p0, nshift;
 };
 hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st);
+
+// one BLAS operation on n right-hand sides in one launch, each with its own
+// operands and first scalar; reductions come back per right-hand side
+struct MultiBlasArgs {
+  const void *x[kMaxRhs];
+  const void *y[kMaxRhs];
+  void *o0[kMaxRhs];
+  void *o1[kMaxRhs];
+  double a[kMaxRhs];
+  int n;
+};
+// partials must hold kMaxBlasBlocks * 3 doubles; out n * nred doubles
+hipError_t blas_multi_launch(const GeomDesc &g, int op, const MultiBlasArgs &m, double *partials, double *out,
+                             hipStream_t st);
 
 }  // namespace qphix_amd

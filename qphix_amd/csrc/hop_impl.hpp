@@ -22,6 +22,7 @@ namespace qphix_amd {
 
 enum HopKind { KD_WILSON = 0, KA_WILSON, KD_CLOVER, KA_CLOVER, KD_TM, KA_TM, KD_FULL, NUM_HOP_KINDS };
 
+
 struct HopArgs {
   const void *psi;
   void *out;
@@ -40,6 +41,13 @@ struct HopArgs {
   // walked z-block by z-block (bz planes), t innermost, so consecutive work-
   // groups of one XCD reuse the source spinor planes they share
   int bz, slabs;
+  // multi-RHS: nrhs > 1 applies the operator to psis[r] -> outs[r] (chis[r]);
+  // the right-hand side is the fastest index of the work-group order, so the
+  // work-groups sharing a site range (and its links and clover) run together
+  int nrhs;
+  const void *psis[kMaxRhs];
+  void *outs[kMaxRhs];
+  const void *chis[kMaxRhs];
 };
 
 // block -> lattice-tile remap so that each XCD (8 per device, round-robin
@@ -122,7 +130,7 @@ __device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3
 }
 
 template <int P, bool C12, int ISIGN, int MU>
-__device__ inline void hop_dim(const HopArgs &a, long s, long sb, long sf, typename PT<P>::calc cb_coef,
+__device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long sb, long sf, typename PT<P>::calc cb_coef,
                                typename PT<P>::calc cf_coef, cx<typename PT<P>::calc> (&acc)[4][3]) {
   typedef typename PT<P>::calc T;
   typedef GaugeLayout<P, C12> GL;
@@ -133,13 +141,13 @@ __device__ inline void hop_dim(const HopArgs &a, long s, long sb, long sf, typen
   if constexpr (GL::per_link) {
     T lv[GL::link_reals];
     // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
-    load_spinor<P>(a.psi, a.vs, sb, sp);
+    load_spinor<P>(psi_p, a.vs, sb, sp);
     load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
     apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
     // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
-    load_spinor<P>(a.psi, a.vs, sf, sp);
+    load_spinor<P>(psi_p, a.vs, sf, sp);
     load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
@@ -147,11 +155,11 @@ __device__ inline void hop_dim(const HopArgs &a, long s, long sb, long sf, typen
   } else {
     T lv[2 * GL::link_reals];
     load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
-    load_spinor<P>(a.psi, a.vs, sb, sp);
+    load_spinor<P>(psi_p, a.vs, sb, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
     apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
-    load_spinor<P>(a.psi, a.vs, sf, sp);
+    load_spinor<P>(psi_p, a.vs, sf, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv + GL::link_reals, U);
     apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tbc, tr, ti, acc);
@@ -243,7 +251,17 @@ template <typename T> __device__ inline void add_ig5(T g, const cx<T> (&in)[4][3
 template <int P, bool C12, int ISIGN, int KIND>
 __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   typedef typename PT<P>::calc T;
-  const long k = xcd_site(blockDim.x);
+  long k = xcd_site(blockDim.x);
+  const void *psi_p = a.psi, *chi_p = a.chi;
+  void *out_p = a.out;
+  if (a.nrhs > 1) {
+    const long lb = k / blockDim.x;
+    const int r = (int)(lb % a.nrhs);
+    k = (lb / a.nrhs) * blockDim.x + threadIdx.x;
+    psi_p = a.psis[r];
+    out_p = a.outs[r];
+    chi_p = a.chis[r];
+  }
   if (k >= a.vh) return;
   long s = k;
   if (a.bz > 0) {
@@ -290,7 +308,7 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   if constexpr (KIND == KA_WILSON || KIND == KA_CLOVER || KIND == KA_TM) {
     T cv[kSpinorReals];
     cx<T> chi[4][3];
-    load_spinor<P, true>(a.chi, a.vs, s, cv);
+    load_spinor<P, true>(chi_p, a.vs, s, cv);
     to_spinor(cv, chi);
     if constexpr (KIND == KA_WILSON) {
       const T al = (T)a.alpha;
@@ -321,10 +339,10 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
       for (int c = 0; c < 3; c++) acc[sp][c] = mk((T)0, (T)0);
   }
 
-  hop_dim<P, C12, ISIGN, 0>(a, s, xb, xf, cs, cs, acc);
-  hop_dim<P, C12, ISIGN, 1>(a, s, yb, yf, cs, cs, acc);
-  hop_dim<P, C12, ISIGN, 2>(a, s, zb, zf, cs, cs, acc);
-  hop_dim<P, C12, ISIGN, 3>(a, s, tb, tf, ctb, ctf, acc);
+  hop_dim<P, C12, ISIGN, 0>(a, psi_p, s, xb, xf, cs, cs, acc);
+  hop_dim<P, C12, ISIGN, 1>(a, psi_p, s, yb, yf, cs, cs, acc);
+  hop_dim<P, C12, ISIGN, 2>(a, psi_p, s, zb, zf, cs, cs, acc);
+  hop_dim<P, C12, ISIGN, 3>(a, psi_p, s, tb, tf, ctb, ctf, acc);
 
   T ov[kSpinorReals];
   if constexpr (KIND == KD_CLOVER) {
@@ -353,7 +371,7 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   } else {
     from_spinor(acc, ov);
   }
-  store_spinor<P, true>(a.out, a.vs, s, ov);
+  store_spinor<P, true>(out_p, a.vs, s, ov);
 }
 
 template <int P>

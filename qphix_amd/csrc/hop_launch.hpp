@@ -7,7 +7,8 @@ namespace qphix_amd {
 
 template <int P, bool C12, int ISIGN>
 static hipError_t launch_hop3(int kind, const HopArgs &a, int block, hipStream_t st) {
-  const dim3 grid((unsigned)((a.vh + block - 1) / block));
+  const int nr = a.nrhs > 1 ? a.nrhs : 1;
+  const dim3 grid((unsigned)(((a.vh + block - 1) / block) * nr));
   switch (kind) {
     case KD_WILSON: hop_kernel<P, C12, ISIGN, KD_WILSON><<<grid, block, 0, st>>>(a); break;
     case KA_WILSON: hop_kernel<P, C12, ISIGN, KA_WILSON><<<grid, block, 0, st>>>(a); break;

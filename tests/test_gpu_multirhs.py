@@ -1,0 +1,97 @@
+"""Multi-right-hand-side kernels and the lockstep multi-RHS CG: every right-hand
+side must give exactly the single-RHS result (same arithmetic per site), and
+match the oracle; the tuned right-hand-side tile is exercised on a lattice
+large enough to be tuned (16^3x64)."""
+import numpy as np
+import pytest
+
+import oracle
+import qphix_amd as q
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("L,prec,c12,n", [([8, 8, 8, 8], q.F64, False, 5),
+                                          ([16, 16, 16, 64], q.F32, True, 6)],
+                         ids=["8^4-f64-n5", "16^3x64-f32c12-n6"])
+def test_multi_clover_dslash_equals_single(L, prec, c12, n):
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    u = oracle.gen_gauge(L, 31, 0.4)
+    clov = oracle.clover_build(L, u, 1.0, 0.05)
+    g = q.Geometry(L, prec, c12)
+    ud = q.pack_gauge(g, u)
+    inv = g.clover().upload(oracle.clover_invert(clov[:vh * 72]))
+    A = g.clover().upload(clov[vh * 72:])
+    D = q.Dslash(g, t_boundary=-1.0, prec_mass_rho=0.02)
+    psi = [g.spinor().upload(oracle.gen_spinor(L, 40 + r, 1), 1) for r in range(n)]
+    chi = [g.spinor().upload(oracle.gen_spinor(L, 60 + r, 1), 1) for r in range(n)]
+    for isign in (1, -1):
+        outs = [g.spinor() for _ in range(n)]
+        D.clov_dslash_multi(outs, psi, ud[0], inv, isign, 0)
+        for r in range(n):
+            one = g.spinor()
+            D.clov_dslash(one, psi[r], ud[0], inv, isign, 0)
+            assert np.array_equal(outs[r].download(0), one.download(0)), r
+        if prec == q.F64:
+            ref = oracle.clover_apply(inv.download(),
+                                      oracle.hop(L, u, psi[2].download(1), isign, 0,
+                                                 oracle.hop_params(-1.0)))
+            assert rel(outs[2].download(0), ref) < 1e-13
+        # achimbdpsi on the other parity, with chi
+        res = [g.spinor() for _ in range(n)]
+        D.clov_achimbdpsi_multi(res, outs, chi, ud[1], A, 0.25, isign, 1)
+        for r in (0, n - 1):
+            one = g.spinor()
+            D.clov_achimbdpsi(one, outs[r], chi[r], ud[1], A, 0.25, isign, 1)
+            assert np.array_equal(res[r].download(1), one.download(1))
+    print(q.tune_summary())
+
+
+def test_multi_operator_and_cg():
+    L = [8, 8, 8, 8]
+    vh = 2048
+    u = oracle.gen_gauge(L, 32, 0.4)
+    clov = oracle.clover_build(L, u, 1.0, 0.05)
+    cl = [clov[:vh * 72], clov[vh * 72:]]
+    g = q.Geometry(L, q.F64, False)
+    ud = q.pack_gauge(g, u)
+    cld = [g.clover().upload(c) for c in cl]
+    invd = [g.clover().upload(oracle.clover_invert(c)) for c in cl]
+    M = q.Operator(g, q.OP_CLOVER, ud, clov=cld, invclov=invd, t_boundary=-1.0)
+    n = 7
+    # point sources (spin x colour at one site) and Gaussian sources
+    rhs_h = []
+    for r in range(4):
+        e = np.zeros(vh * 24)
+        e[(123 * 24) + 2 * (3 * (r % 4) + r % 3)] = 1.0
+        rhs_h.append(e)
+    rhs_h += [oracle.gen_spinor(L, 70 + r, 1) for r in range(n - 4)]
+    rhs = [g.spinor().upload(h, 1) for h in rhs_h]
+    # operator: multi equals single
+    outs = [g.spinor() for _ in range(n)]
+    M.apply_multi(outs, rhs, 1, 1)
+    for r in range(n):
+        one = g.spinor()
+        M(one, rhs[r], 1, 1)
+        assert np.array_equal(outs[r].download(1), one.download(1))
+    # lockstep CG vs one solve per right-hand side
+    x = [g.spinor() for _ in range(n)]
+    sts = q.InvCG(M, 1000).solve_multi(x, rhs, 1e-10)
+    hp = oracle.hop_params(-1.0)
+    Mo = oracle.Operator(oracle.CLOVER, L, u, hp, clov=cl, invclov=[invd[0].download(), invd[1].download()])
+    for r in range(n):
+        assert sts[r].status == q.OK
+        xs = g.spinor()
+        st1 = q.InvCG(M, 1000)(xs, rhs[r], 1e-10)
+        assert st1.n_iters == sts[r].n_iters
+        assert np.array_equal(xs.download(1), x[r].download(1))
+        xh = x[r].download(1)
+        res = rhs_h[r] - Mo(Mo(xh, 1, 1), -1, 1)
+        assert np.linalg.norm(res) / np.linalg.norm(rhs_h[r]) < 1.5e-10
+    norms = q.norm2_multi(g, rhs)
+    for r in range(n):
+        assert abs(norms[r] - rhs_h[r] @ rhs_h[r]) <= 1e-13 * norms[r]
