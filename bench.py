@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--hop-config", default=None,
                     help="block,bz,slabs: fixed hop-kernel launch configuration instead of "
                          "autotuning (used for profiler runs so every launch is identical)")
@@ -173,6 +174,8 @@ def main():
 
     if not args.no_solve:
         extra["solve"] = solve_compare(q, g64, u64)
+    if not args.no_batch:
+        extra["batch_tmclover"] = batch_compare(q)
 
     result = {
         "metric": "clover Dslash GB/s (% HBM peak); solve s to 1e-10, mixed vs FP64",
@@ -259,6 +262,72 @@ def solve_compare(q, g64, u64):
     out["speedup"] = round(t64 / tm, 3)
     log(f"solve: fp64 {t64:.2f} s ({st.n_iters} it), mixed {tm:.2f} s, speedup {t64 / tm:.2f}")
     return out
+
+
+def batch_compare(q):
+    """Uneven batch of twisted-mass clover solves: lattices 8^4, 12^3x24,
+    16^3x32, 24^3x48 (own gauge seed, mu ~ U[0.001, 0.01]), 4 Gaussian sources
+    each; FP16-storage inner CG on M^+M, FP64 outer refinement to 1e-10.
+    Batched (one launch per step for all 16 systems, graph-replayed CG
+    iterations) against solving the systems one by one."""
+    import random
+    lats = [[8, 8, 8, 8], [12, 12, 12, 24], [16, 16, 16, 32], [24, 24, 24, 48]]
+    mass = 1.0 / (2 * 0.1240) - 4.0
+    rng = random.Random(5)
+    keep, ops64, ops16, rhs, gs = [], [], [], [], []
+    for i, L in enumerate(lats):
+        mu = rng.uniform(0.001, 0.01)
+        g64 = q.Geometry(L, q.F64, False)
+        g16 = q.Geometry(L, q.F16, True)
+        u = q.gen_gauge(g64, 5 + i, 0.4)
+        A = q.clover_build(g64, u, CSW, mass, 1, q.CLOV_A)
+        fp = q.clover_build(g64, u, CSW, mass, 0, q.CLOV_TMINV, mu)
+        fm = q.clover_build(g64, u, CSW, mass, 0, q.CLOV_TMINV, -mu)
+        u16 = q.convert_gauge(g16, g64, u)
+        A16 = q.convert_clover(g16, g64, A)
+        fp16 = q.convert_clover(g16, g64, fp, full=True)
+        fm16 = q.convert_clover(g16, g64, fm, full=True)
+        kw = dict(twisted_mass=mu, prec_mass_rho=mu, t_boundary=-1.0)
+        M64 = q.Operator(g64, q.OP_TMCLOVER, u, clov=[None, A], fullclov=[[fp, fm], None], **kw)
+        M16 = q.Operator(g16, q.OP_TMCLOVER, u16, clov=[None, A16], fullclov=[[fp16, fm16], None], **kw)
+        keep += [g64, g16, u, A, fp, fm, u16, A16, fp16, fm16, M64, M16]
+        for r in range(4):
+            ops64.append(M64)
+            ops16.append(M16)
+            gs.append(g64)
+            rhs.append(q.gen_spinor(g64, 100 * i + r, 1))
+    # warm-up: first applications tune the hop kernels of each lattice
+    for M, g in zip(ops64[::4] + ops16[::4], gs[::4] + [o.geom for o in ops16[::4]]):
+        a, b = g.spinor(), g.spinor()
+        M(a, b, 1, 1)
+        M(a, b, -1, 1)
+    q.sync()
+    B = q.Batch(ops16)
+    xb = [g.spinor() for g in gs]
+    B.richardson(ops64, xb, rhs, 1e-10, delta=1e-2, graph_steps=8)  # untimed warm-up
+    xb = [g.spinor() for g in gs]
+    q.sync()
+    t0 = time.perf_counter()
+    stb = B.richardson(ops64, xb, rhs, 1e-10, delta=1e-2, graph_steps=8)
+    q.sync()
+    tb = time.perf_counter() - t0
+    xs = [g.spinor() for g in gs]
+    solvers = []
+    for M64, M16 in zip(ops64, ops16):
+        inner = q.InvCG(M16, 2000)
+        solvers.append(q.InvRichardsonMultiPrec(M64, inner, 1e-2, 50, mdagm=True))
+    q.sync()
+    t0 = time.perf_counter()
+    sts = [sv(x, b, 1e-10) for sv, x, b in zip(solvers, xs, rhs)]
+    q.sync()
+    ts = time.perf_counter() - t0
+    ok_b = all(st.status == q.OK for st in stb)
+    ok_s = all(st.status == q.OK for st in sts)
+    worst = max(st.rsd_sq_final for st in stb) ** 0.5
+    log(f"batch tm-clover: batched {tb:.3f} s, one-by-one {ts:.3f} s, speedup {ts / tb:.2f}")
+    return {"lattices": lats, "rhs_per_lattice": 4, "batched_seconds": round(tb, 4),
+            "one_by_one_seconds": round(ts, 4), "speedup": round(ts / tb, 3),
+            "all_converged": ok_b and ok_s, "worst_true_rel_resid_batched": worst}
 
 
 if __name__ == "__main__":

@@ -276,6 +276,25 @@ int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[]
                        double *rsd_sq_final, unsigned long *site_flops,
                        unsigned long *mv_apps);
 
+/* ---- batched solves (no reference counterpart): several single-flavour
+ *      systems of one operator kind and precision, possibly on lattices of
+ *      different sizes, advanced together. Each step is one persistent launch
+ *      over all unconverged systems; scalars stay on the device in FP64; with
+ *      graph_steps > 0 one CG iteration is captured in a HIP graph and replayed
+ *      graph_steps times between host convergence checks. */
+typedef struct qphix_batch qphix_batch;
+int qphix_batch_create(qphix_op *const ops[], int nsys, qphix_batch **b);
+int qphix_batch_destroy(qphix_batch *b);
+/* CG on M^+M per system (invcg.h iteration); relative targets rsd_target[nsys] */
+int qphix_batch_cg(qphix_batch *b, void *const x[], const void *const rhs[], const double rsd_target[],
+                   int max_iters, int isign, int target_cb, int graph_steps, qphix_solve_stats st[]);
+/* FP64 defect correction (outer[i] on the same lattice as the batch's
+ * operator i) around the batched inner CG on M^+M, relaxation delta */
+int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[], void *const x[],
+                           const void *const rhs[], double rsd_target, double delta, int max_outer,
+                           int max_inner, int isign, int target_cb, int graph_steps,
+                           qphix_solve_stats st[]);
+
 /* ---- fused BLAS: blas_new_c.h functions with num_flav = 1 (line numbers).
  *      Scalars are doubles, applied in the geometry's arithmetic type;
  *      reductions accumulate in FP64 and return on the host. ------------- */

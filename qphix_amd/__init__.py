@@ -463,6 +463,40 @@ class MInvCG:
         return nit.value, list(fin), mv.value
 
 
+class Batch:
+    """Several single-flavour systems solved together (batched launches,
+    device scalars, optional HIP-graph replay of the CG iteration)."""
+
+    def __init__(self, ops):
+        self.ops = list(ops)
+        h = VP()
+        call("qphix_batch_create", (VP * len(ops))(*[o.h for o in ops]), len(ops), C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        try:
+            lib().qphix_batch_destroy(self.h)
+        except Exception:
+            pass
+
+    def cg(self, x, rhs, rsd, max_iters=1000, isign=1, target_cb=1, graph_steps=0):
+        n = len(self.ops)
+        st = (SolveStats * n)()
+        tg = rsd if isinstance(rsd, (list, tuple)) else [rsd] * n
+        call("qphix_batch_cg", self.h, _pa(x), _pa(rhs), darr(tg), max_iters, isign, target_cb,
+             graph_steps, st)
+        return list(st)
+
+    def richardson(self, outer_ops, x, rhs, rsd, delta=1e-2, max_outer=50, max_inner=2000,
+                   isign=1, target_cb=1, graph_steps=0):
+        n = len(self.ops)
+        st = (SolveStats * n)()
+        call("qphix_batch_richardson", self.h, (VP * n)(*[o.h for o in outer_ops]), _pa(x), _pa(rhs),
+             C.c_double(rsd), C.c_double(delta), max_outer, max_inner, isign, target_cb,
+             graph_steps, st)
+        return list(st)
+
+
 # ---------------------------------------------------------------------------
 # BLAS (blas_new_c.h), single flavour
 def norm2(geom, x):

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/qphix_amd.h"
+#include "batch.hpp"
 #include "fields.hpp"
 #include "hop_impl.hpp"
 
@@ -25,6 +26,15 @@ hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int 
 extern template hipError_t launch_hop_prec<P64>(int, bool, int, const HopArgs &, int, hipStream_t);
 extern template hipError_t launch_hop_prec<P32>(int, bool, int, const HopArgs &, int, hipStream_t);
 extern template hipError_t launch_hop_prec<P16>(int, bool, int, const HopArgs &, int, hipStream_t);
+template <int P>
+hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
+                                 const int *active, int grid, hipStream_t st);
+extern template hipError_t launch_hop_batch_prec<P64>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                      int, hipStream_t);
+extern template hipError_t launch_hop_batch_prec<P32>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                      int, hipStream_t);
+extern template hipError_t launch_hop_batch_prec<P16>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                      int, hipStream_t);
 }  // namespace qphix_amd
 
 struct qphix_geom {
@@ -1722,4 +1732,385 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
   for (void *w : s->work) qphix_free(w);
   s->work.clear();
   return convP ? QPHIX_OK : fail(QPHIX_ENOCONV, "multishift CG did not converge in %d iterations", k);
+}
+
+// ---------------------------------------------------------------------------
+// batched solves: many single-flavour systems (possibly on different lattices)
+// advanced together, one launch per step for all of them, with device-resident
+// FP64 scalars and (optionally) the iteration captured once in a HIP graph
+// ---------------------------------------------------------------------------
+struct qphix_batch {
+  std::vector<qphix_op *> ops;
+  int n = 0, prec = 0, c12 = 0, kind = 0;
+  std::vector<void *> tmp, mp, mmp, p, r;  // per-system work spinors
+  std::vector<int> off;                    // chunk / work-group offsets (256 sites)
+  int *d_off = nullptr, *d_active = nullptr;
+  HopSeg *d_hop = nullptr;                 // 4 stages x n
+  SegVec *d_seg = nullptr;                 // 8 step kinds x n
+  CgState *d_state = nullptr;
+  double *d_alpha = nullptr, *d_beta = nullptr, *d_d = nullptr, *d_cp = nullptr, *d_rsd = nullptr;
+  double *d_partials = nullptr;
+};
+
+static int batch_free_dev(qphix_batch *b) {
+  void *ptrs[] = {b->d_off, b->d_active, b->d_hop, b->d_seg, b->d_state, b->d_alpha, b->d_beta,
+                  b->d_d, b->d_cp, b->d_rsd, b->d_partials};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_batch_destroy(qphix_batch *b) {
+  if (!b) return QPHIX_OK;
+  if (g_init) (void)hipStreamSynchronize(g_stream);
+  std::vector<void *> *sets[5] = {&b->tmp, &b->mp, &b->mmp, &b->p, &b->r};
+  for (auto *v : sets)
+    for (void *q : *v) qphix_free(q);
+  batch_free_dev(b);
+  delete b;
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_batch_create(qphix_op *const ops[], int nsys, qphix_batch **out) {
+  ENSURE_INIT();
+  CHKPTR(ops); CHKPTR(out);
+  if (nsys < 1) return fail(QPHIX_EDOMAIN, "batch of %d systems", nsys);
+  qphix_batch *b = new qphix_batch();
+  b->n = nsys;
+  for (int i = 0; i < nsys; i++) {
+    if (!ops[i]) { delete b; return fail(QPHIX_EINVAL, "null operator %d", i); }
+    if (ops[i]->nflav != 1) { delete b; return fail(QPHIX_EDOMAIN, "batched solves take single-flavour operators"); }
+    b->ops.push_back(ops[i]);
+  }
+  const GeomDesc &g0 = ops[0]->g->d;
+  b->prec = g0.prec;
+  b->c12 = g0.c12;
+  b->kind = ops[0]->p.kind;
+  for (int i = 1; i < nsys; i++) {
+    const GeomDesc &gi = ops[i]->g->d;
+    if (gi.prec != b->prec || gi.c12 != b->c12 || ops[i]->p.kind != b->kind) {
+      delete b;
+      return fail(QPHIX_EDOMAIN, "batched operators must share precision, compression and kind");
+    }
+  }
+  b->off.assign(nsys + 1, 0);
+  for (int i = 0; i < nsys; i++) b->off[i + 1] = b->off[i] + (int)((ops[i]->g->d.vh + 255) / 256);
+  std::vector<void *> *sets[5] = {&b->tmp, &b->mp, &b->mmp, &b->p, &b->r};
+  for (auto *v : sets)
+    for (int i = 0; i < nsys; i++) {
+      void *q = nullptr;
+      int rc = qphix_alloc_spinor(ops[i]->g, &q);
+      if (rc) { qphix_batch_destroy(b); return rc; }
+      v->push_back(q);
+    }
+  const size_t nn = nsys;
+  bool ok = hipMalloc(&b->d_off, sizeof(int) * (nn + 1)) == hipSuccess &&
+            hipMalloc(&b->d_active, sizeof(int) * nn) == hipSuccess &&
+            hipMalloc(&b->d_hop, sizeof(HopSeg) * 4 * nn) == hipSuccess &&
+            hipMalloc(&b->d_seg, sizeof(SegVec) * 8 * nn) == hipSuccess &&
+            hipMalloc(&b->d_state, sizeof(CgState) * nn) == hipSuccess &&
+            hipMalloc(&b->d_alpha, sizeof(double) * nn) == hipSuccess &&
+            hipMalloc(&b->d_beta, sizeof(double) * nn) == hipSuccess &&
+            hipMalloc(&b->d_d, sizeof(double) * nn) == hipSuccess &&
+            hipMalloc(&b->d_cp, sizeof(double) * nn) == hipSuccess &&
+            hipMalloc(&b->d_rsd, sizeof(double) * nn) == hipSuccess &&
+            hipMalloc(&b->d_partials, sizeof(double) * (size_t)b->off[nsys]) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    qphix_batch_destroy(b);
+    return fail(QPHIX_ENOMEM, "batch device allocation failed");
+  }
+  if (hipMemcpy(b->d_off, b->off.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice) != hipSuccess) {
+    qphix_batch_destroy(b);
+    return fail(QPHIX_EHIP, "batch offsets upload failed");
+  }
+  *out = b;
+  return QPHIX_OK;
+}
+
+// the two hop launches of op (isign, target t): in -> tmp (other cb) -> out
+static int op_stages(qphix_op *op, int isign, int t, const void *in, void *tmp, void *out, int kinds[2],
+                     HopArgs args[2]) {
+  const int o = 1 - t;
+  const qphix_op_params &p = op->p;
+  qphix_dslash *D = &op->d;
+  switch (p.kind) {
+    case QPHIX_OP_WILSON: {
+      const double al = 4.0 + p.mass;
+      kinds[0] = KD_WILSON;
+      args[0] = make_hop_args(D, o, p.u[o], nullptr, 1.0, 0, 0, 0, 0);
+      kinds[1] = KA_WILSON;
+      args[1] = make_hop_args(D, t, p.u[t], nullptr, -0.25 / al, al, 0, 0, 0);
+    } break;
+    case QPHIX_OP_CLOVER:
+      kinds[0] = KD_CLOVER;
+      args[0] = make_hop_args(D, o, p.u[o], p.invclov[o], 1.0, 0, 0, 0, 0);
+      kinds[1] = KA_CLOVER;
+      args[1] = make_hop_args(D, t, p.u[t], p.clov[t], -0.25, 0, 0, 0, D->p.prec_mass_rho);
+      break;
+    case QPHIX_OP_TMWILSON: {
+      const double al = 4.0 + p.mass, mu = p.twisted_mass;
+      kinds[0] = KD_TM;
+      args[0] = make_hop_args(D, o, p.u[o], nullptr, 1.0, 0, mu / al, al / (mu * mu + al * al), 0);
+      kinds[1] = KA_TM;
+      args[1] = make_hop_args(D, t, p.u[t], nullptr, -0.25, al, mu / al, 0, 0);
+    } break;
+    case QPHIX_OP_TMCLOVER: {
+      const int clidx = isign == 1 ? 0 : 1;
+      kinds[0] = KD_FULL;
+      args[0] = make_hop_args(D, o, p.u[o], p.fullclov[o][clidx], 1.0, 0, 0, 0, 0);
+      kinds[1] = KA_CLOVER;
+      args[1] = make_hop_args(D, t, p.u[t], p.clov[t], -0.25, 0, 0, 0, D->p.prec_mass_rho);
+    } break;
+    default: return fail(QPHIX_EDOMAIN, "batched solves support Wilson, clover and twisted-mass kinds");
+  }
+  if ((kinds[0] == KD_CLOVER || kinds[0] == KD_FULL) && !args[0].clov) return fail(QPHIX_EINVAL, "clover field missing");
+  if (kinds[1] == KA_CLOVER && !args[1].clov) return fail(QPHIX_EINVAL, "clover field missing");
+  args[0].psi = in;
+  args[0].out = tmp;
+  args[1].psi = tmp;
+  args[1].chi = in;
+  args[1].out = out;
+  return QPHIX_OK;
+}
+
+static int batch_grid(const qphix_batch *b) {
+  const int total = b->off[b->n];
+  return total < 2048 ? total : 2048;
+}
+
+static hipError_t batch_hop(const qphix_batch *b, int stage, int kind, int isign) {
+  const HopSeg *segs = b->d_hop + (size_t)stage * b->n;
+  switch (b->prec) {
+    case P64: return launch_hop_batch_prec<P64>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
+    case P32: return launch_hop_batch_prec<P32>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
+    default: return launch_hop_batch_prec<P16>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
+  }
+}
+static hipError_t batch_blas(const qphix_batch *b, int step, int op, const double *scal, double *out, bool gate) {
+  return batch_blas_launch(b->prec, op, b->d_seg + (size_t)step * b->n, b->d_off, b->n, gate ? b->d_active : nullptr,
+                           scal, b->d_partials, out, batch_grid(b), g_stream);
+}
+
+// one CG iteration on M^+M for every active system (no host interaction)
+static hipError_t batch_cg_iteration(const qphix_batch *b, const int kinds[4], int isign) {
+  hipError_t e;
+  if ((e = batch_hop(b, 0, kinds[0], isign)) != hipSuccess) return e;
+  if ((e = batch_hop(b, 1, kinds[1], isign)) != hipSuccess) return e;
+  if ((e = batch_hop(b, 2, kinds[2], -isign)) != hipSuccess) return e;
+  if ((e = batch_hop(b, 3, kinds[3], -isign)) != hipSuccess) return e;
+  if ((e = batch_blas(b, 0, BB_NORM2, nullptr, b->d_d, true)) != hipSuccess) return e;         // d = |mp|^2
+  if ((e = batch_cg_alpha(b->d_state, b->d_d, b->d_alpha, b->d_active, b->n, g_stream)) != hipSuccess) return e;
+  if ((e = batch_blas(b, 1, BB_RMAMMP, b->d_alpha, b->d_cp, true)) != hipSuccess) return e;   // r -= a mmp, x += a p
+  if ((e = batch_cg_update(b->d_state, b->d_cp, b->d_beta, b->d_active, b->n, g_stream)) != hipSuccess) return e;
+  return batch_blas(b, 2, BB_AYPX, b->d_beta, nullptr, true);                                   // p = r + b p
+}
+
+extern "C" int qphix_batch_cg(qphix_batch *b, void *const x[], const void *const rhs[], const double rsd_target[],
+                              int max_iters, int isign, int t, int graph_steps, qphix_solve_stats st[]) {
+  ENSURE_INIT();
+  CHKPTR(b); CHKPTR(x); CHKPTR(rhs); CHKPTR(rsd_target); CHKPTR(st);
+  TRY(check_cb(t));
+  if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1");
+  const int n = b->n;
+  // segment tables for this solve's vectors
+  std::vector<HopSeg> hop(4 * (size_t)n);
+  std::vector<SegVec> seg(8 * (size_t)n);
+  int kinds[4];
+  for (int i = 0; i < n; i++) {
+    if (!x[i] || !rhs[i]) return fail(QPHIX_EINVAL, "null vector for system %d", i);
+    const GeomDesc &g = b->ops[i]->g->d;
+    HopArgs a[2];
+    int k2[2];
+    TRY(op_stages(b->ops[i], isign, t, b->p[i], b->tmp[i], b->mp[i], k2, a));   // mp = M p
+    hop[0 * n + i] = HopSeg{a[0], i};
+    hop[1 * n + i] = HopSeg{a[1], i};
+    kinds[0] = k2[0];
+    kinds[1] = k2[1];
+    TRY(op_stages(b->ops[i], -isign, t, b->mp[i], b->tmp[i], b->mmp[i], k2, a));  // mmp = M^+ mp
+    hop[2 * n + i] = HopSeg{a[0], i};
+    hop[3 * n + i] = HopSeg{a[1], i};
+    kinds[2] = k2[0];
+    kinds[3] = k2[1];
+    auto sv = [&](const void *xx, const void *yy, void *o0, void *o1) { return SegVec{xx, yy, o0, o1, g.vh, g.vs, i}; };
+    seg[0 * n + i] = sv(b->mp[i], nullptr, nullptr, nullptr);        // |mp|^2
+    seg[1 * n + i] = sv(b->mmp[i], b->p[i], b->r[i], x[i]);          // r -= a mmp, x += a p
+    seg[2 * n + i] = sv(b->r[i], nullptr, b->p[i], nullptr);         // p = r + b p
+    seg[3 * n + i] = sv(rhs[i], nullptr, nullptr, nullptr);          // |rhs|^2
+    seg[4 * n + i] = sv(rhs[i], b->mmp[i], b->r[i], nullptr);        // r = rhs - mmp
+    seg[5 * n + i] = sv(b->r[i], nullptr, b->p[i], nullptr);         // p = r
+    seg[6 * n + i] = sv(x[i], nullptr, nullptr, nullptr);            // |x|^2
+  }
+  HIPCHK(hipMemcpyAsync(b->d_hop, hop.data(), sizeof(HopSeg) * hop.size(), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(b->d_seg, seg.data(), sizeof(SegVec) * seg.size(), hipMemcpyHostToDevice, g_stream));
+  // set-up: r = rhs - M^+M x (the operator only where x != 0), p = r, targets
+  std::vector<double> rn(n), xn(n), rsd(n);
+  HIPCHK(batch_blas(b, 3, BB_NORM2, nullptr, b->d_d, false));
+  HIPCHK(hipMemcpyAsync(rn.data(), b->d_d, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(batch_blas(b, 6, BB_NORM2, nullptr, b->d_d, false));
+  HIPCHK(hipMemcpyAsync(xn.data(), b->d_d, sizeof(double) * n, hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  std::vector<int> act(n);
+  for (int i = 0; i < n; i++) {
+    rsd[i] = rsd_target[i] * rsd_target[i] * rn[i];
+    act[i] = xn[i] != 0.0;
+  }
+  HIPCHK(hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * n, hipMemcpyHostToDevice, g_stream));
+  // hop stages 0-3 read p; start from p = x for the systems with a guess
+  std::vector<SegVec> cpx(n);
+  for (int i = 0; i < n; i++) cpx[i] = seg[6 * n + i], cpx[i].o0 = b->p[i];
+  HIPCHK(hipMemcpyAsync(b->d_seg + 7 * (size_t)n, cpx.data(), sizeof(SegVec) * n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(batch_blas(b, 7, BB_COPY, nullptr, nullptr, true));
+  for (int s = 0; s < 4; s++) HIPCHK(batch_hop(b, s, kinds[s], s < 2 ? isign : -isign));
+  // systems without a guess: mmp = 0 so that r = rhs
+  std::vector<SegVec> zero(n);
+  for (int i = 0; i < n; i++) zero[i] = SegVec{nullptr, nullptr, b->mmp[i], nullptr, seg[i].vh, seg[i].vs, i};
+  for (int i = 0; i < n; i++) act[i] = !act[i];
+  HIPCHK(hipMemcpyAsync(b->d_seg + 7 * (size_t)n, zero.data(), sizeof(SegVec) * n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(batch_blas(b, 7, BB_ZERO, nullptr, nullptr, true));
+  HIPCHK(batch_blas(b, 4, BB_XMYNORM2, nullptr, b->d_cp, false));
+  HIPCHK(batch_blas(b, 5, BB_COPY, nullptr, nullptr, false));
+  HIPCHK(hipMemcpyAsync(b->d_rsd, rsd.data(), sizeof(double) * n, hipMemcpyHostToDevice, g_stream));
+  HIPCHK(batch_cg_init(b->d_state, b->d_cp, b->d_rsd, b->d_active, max_iters, n, g_stream));
+  const int nhops = 4;
+  (void)nhops;
+  // iterations
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (graph_steps > 0) {
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = batch_cg_iteration(b, kinds, isign);
+    hipError_t e2 = hipStreamEndCapture(g_stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(QPHIX_EHIP, "graph capture of the CG iteration failed: %s",
+                  hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(QPHIX_EHIP, "graph instantiation failed");
+    }
+  }
+  const int check = graph_steps > 0 ? graph_steps : 1;
+  int done_iters = 0;
+  int rc = QPHIX_OK;
+  while (done_iters <= max_iters + 1) {
+    for (int k = 0; k < check; k++) {
+      hipError_t e = exec ? hipGraphLaunch(exec, g_stream) : batch_cg_iteration(b, kinds, isign);
+      if (e != hipSuccess) {
+        rc = fail(QPHIX_EHIP, "batched CG iteration: %s", hipGetErrorString(e));
+        break;
+      }
+    }
+    if (rc) break;
+    done_iters += check;
+    if (hipMemcpyAsync(act.data(), b->d_active, sizeof(int) * n, hipMemcpyDeviceToHost, g_stream) != hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess) {
+      rc = fail(QPHIX_EHIP, "reading convergence flags failed");
+      break;
+    }
+    bool any = false;
+    for (int i = 0; i < n; i++) any |= act[i] != 0;
+    if (!any) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  std::vector<CgState> hs(n);
+  HIPCHK(hipMemcpy(hs.data(), b->d_state, sizeof(CgState) * n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) {
+    memset(&st[i], 0, sizeof(st[i]));
+    st[i].n_iters = hs[i].iters;
+    st[i].rsd_sq_final = hs[i].cp;
+    st[i].mv_apps = 2 * (unsigned long)(hs[i].iters + 1);
+    st[i].status = hs[i].converged ? QPHIX_OK : QPHIX_ENOCONV;
+  }
+  return QPHIX_OK;
+}
+
+// FP64 defect correction around a batched inner CG on M^+M
+// (inv_richardson_multiprec.h:42-239 per system with MdagM = true)
+extern "C" int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[], void *const x[],
+                                      const void *const rhs[], double rsd_target, double delta, int max_outer,
+                                      int max_inner, int isign, int t, int graph_steps, qphix_solve_stats st[]) {
+  ENSURE_INIT();
+  CHKPTR(inner); CHKPTR(outer); CHKPTR(x); CHKPTR(rhs); CHKPTR(st);
+  const int n = inner->n;
+  for (int i = 0; i < n; i++) {
+    CHKPTR(outer[i]);
+    TRY(same_lattice(outer[i]->g, inner->ops[i]->g));
+  }
+  std::vector<void *> r(n), dx(n), tmp(n), tmp2(n), ri(n), dxi(n);
+  struct Guard {
+    std::vector<void *> *v[6];
+    ~Guard() {
+      for (auto *q : v)
+        for (void *p : *q) qphix_free(p);
+    }
+  } guard{{&r, &dx, &tmp, &tmp2, &ri, &dxi}};
+  for (int i = 0; i < n; i++) {
+    TRY(qphix_alloc_spinor(outer[i]->g, &r[i]));
+    TRY(qphix_alloc_spinor(outer[i]->g, &dx[i]));
+    TRY(qphix_alloc_spinor(outer[i]->g, &tmp[i]));
+    TRY(qphix_alloc_spinor(outer[i]->g, &tmp2[i]));
+    TRY(qphix_alloc_spinor(inner->ops[i]->g, &ri[i]));
+    TRY(qphix_alloc_spinor(inner->ops[i]->g, &dxi[i]));
+  }
+  std::vector<double> rhs_sq(n), rn2(n), rsd_t(n), tgt(n);
+  std::vector<int> outer_it(n, 0);
+  std::vector<unsigned long> mv(n, 0);
+  auto mdagm = [&](int i, const void *in) -> int {
+    void *a[1] = {tmp2[i]}, *c[1] = {tmp[i]};
+    const void *ii[1] = {in}, *mm[1] = {tmp2[i]};
+    TRY(qphix_op_apply(outer[i], a, ii, isign, t));
+    TRY(qphix_op_apply(outer[i], c, mm, -isign, t));
+    mv[i] += 2;
+    return QPHIX_OK;
+  };
+  for (int i = 0; i < n; i++) {
+    TRY(qphix_norm2_spinor(outer[i]->g, rhs[i], &rhs_sq[i]));
+    rsd_t[i] = rsd_target * rsd_target * rhs_sq[i];
+    TRY(mdagm(i, x[i]));
+    TRY(qphix_xmy_norm2(outer[i]->g, r[i], rhs[i], tmp[i], &rn2[i]));
+  }
+  for (int it = 1; it <= max_outer; it++) {
+    bool any = false;
+    for (int i = 0; i < n; i++) {
+      const bool a = rn2[i] >= rsd_t[i];
+      any |= a;
+      const double rn = sqrt(rn2[i]);
+      if (a) {
+        const double dfin = sqrt(rsd_t[i] / rn2[i]);
+        tgt[i] = dfin > delta ? dfin : delta;
+        TRY(qphix_convert_spinor(inner->ops[i]->g, ri[i], 1.0 / rn, outer[i]->g, r[i]));
+        outer_it[i] = it;
+      } else {
+        tgt[i] = 1e150;
+        TRY(qphix_zero_spinor(inner->ops[i]->g, ri[i]));
+      }
+      TRY(qphix_zero_spinor(inner->ops[i]->g, dxi[i]));
+    }
+    if (!any) break;
+    std::vector<qphix_solve_stats> ist(n);
+    TRY(qphix_batch_cg(inner, dxi.data(), (const void *const *)ri.data(), tgt.data(), max_inner, isign, t, graph_steps,
+                       ist.data()));
+    for (int i = 0; i < n; i++) {
+      if (rn2[i] < rsd_t[i]) continue;
+      mv[i] += ist[i].mv_apps;
+      TRY(qphix_convert_spinor(outer[i]->g, dx[i], sqrt(rn2[i]), inner->ops[i]->g, dxi[i]));
+      TRY(mdagm(i, dx[i]));
+      TRY(qphix_richardson_rxupdate_norm_r(outer[i]->g, x[i], r[i], dx[i], tmp[i], &rn2[i]));
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    TRY(mdagm(i, x[i]));
+    TRY(qphix_xmy_norm2(outer[i]->g, r[i], rhs[i], tmp[i], &rn2[i]));
+    memset(&st[i], 0, sizeof(st[i]));
+    st[i].n_iters = outer_it[i];
+    st[i].rsd_sq_final = rn2[i] / rhs_sq[i];
+    st[i].mv_apps = mv[i];
+    st[i].status = rn2[i] < rsd_t[i] ? QPHIX_OK : QPHIX_ENOCONV;
+  }
+  return QPHIX_OK;
 }

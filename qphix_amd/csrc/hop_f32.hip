@@ -3,4 +3,6 @@
 
 namespace qphix_amd {
 template hipError_t launch_hop_prec<P32>(int, bool, int, const HopArgs &, int, hipStream_t);
+template hipError_t launch_hop_batch_prec<P32>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                  int, hipStream_t);
 }

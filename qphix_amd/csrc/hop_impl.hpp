@@ -52,11 +52,10 @@ struct HopArgs {
 
 // block -> lattice-tile remap so that each XCD (8 per device, round-robin
 // dispatch) walks one contiguous range of sites and keeps its L2 warm
-__device__ inline long xcd_site(int block_threads) {
+__device__ inline int xcd_block() {
   const int nb = gridDim.x, bid = blockIdx.x;
   const int q = nb >> 3, r = nb & 7, xcd = bid & 7, loc = bid >> 3;
-  const int lb = (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
-  return (long)lb * block_threads + threadIdx.x;
+  return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
 }
 
 template <typename T> __device__ inline void to_spinor(const T *v, cx<T> (&p)[4][3]) {
@@ -248,21 +247,10 @@ template <typename T> __device__ inline void add_ig5(T g, const cx<T> (&in)[4][3
   }
 }
 
+// the hop for traversal index k (< a.vh) of one right-hand side
 template <int P, bool C12, int ISIGN, int KIND>
-__global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
+__device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, void *out_p, const void *chi_p) {
   typedef typename PT<P>::calc T;
-  long k = xcd_site(blockDim.x);
-  const void *psi_p = a.psi, *chi_p = a.chi;
-  void *out_p = a.out;
-  if (a.nrhs > 1) {
-    const long lb = k / blockDim.x;
-    const int r = (int)(lb % a.nrhs);
-    k = (lb / a.nrhs) * blockDim.x + threadIdx.x;
-    psi_p = a.psis[r];
-    out_p = a.outs[r];
-    chi_p = a.chis[r];
-  }
-  if (k >= a.vh) return;
   long s = k;
   if (a.bz > 0) {
     const int plane = a.nxh * a.ny;
@@ -372,6 +360,52 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
     from_spinor(acc, ov);
   }
   store_spinor<P, true>(out_p, a.vs, s, ov);
+}
+
+template <int P, bool C12, int ISIGN, int KIND>
+__global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
+  int lb = xcd_block();
+  const void *psi_p = a.psi, *chi_p = a.chi;
+  void *out_p = a.out;
+  if (a.nrhs > 1) {
+    const int r = lb % a.nrhs;
+    lb /= a.nrhs;
+    // The per-RHS pointers are read in place from the kernel-argument segment,
+    // where this struct (the kernel's only parameter) starts at offset 0:
+    // indexing the by-value parameter with a run-time index would make the
+    // compiler copy the whole struct to private memory for every lane.
+    const HopArgs *ka = (const HopArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    psi_p = ka->psis[r];
+    out_p = ka->outs[r];
+    chi_p = ka->chis[r];
+  }
+  const long k = (long)lb * blockDim.x + threadIdx.x;
+  if (k >= a.vh) return;
+  hop_body<P, C12, ISIGN, KIND>(a, k, psi_p, out_p, chi_p);
+}
+
+// Batched hop: one launch covers several systems (possibly on different
+// lattices). Segment g owns the work-groups [blk_off[g], blk_off[g+1]) of a
+// flat index space that a fixed grid walks with a block stride (persistent
+// form), so the launch shape does not depend on the batch and can be captured
+// once in a graph. Segments whose system has converged (active[sys] == 0) are
+// skipped.
+struct HopSeg {
+  HopArgs a;
+  int sys;
+};
+template <int P, bool C12, int ISIGN, int KIND>
+__global__ void __launch_bounds__(256) hop_batch_kernel(const HopSeg *segs, const int *blk_off, int nseg,
+                                                         const int *active) {
+  const int total = blk_off[nseg];
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    int g = 0;
+    while (g + 1 < nseg && blk_off[g + 1] <= b) g++;
+    const HopSeg &sg = segs[g];
+    if (active && !active[sg.sys]) continue;
+    const long k = (long)(b - blk_off[g]) * blockDim.x + threadIdx.x;
+    if (k < sg.a.vh) hop_body<P, C12, ISIGN, KIND>(sg.a, k, sg.a.psi, sg.a.out, sg.a.chi);
+  }
 }
 
 template <int P>

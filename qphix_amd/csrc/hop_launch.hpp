@@ -32,4 +32,33 @@ hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int 
   return hipErrorInvalidValue;
 }
 
+constexpr int kBatchBlock = 256;
+
+template <int P, bool C12, int ISIGN>
+static hipError_t launch_batch3(int kind, const HopSeg *segs, const int *off, int nseg, const int *active, int grid,
+                                hipStream_t st) {
+  switch (kind) {
+#define BK(K) \
+  case K: hop_batch_kernel<P, C12, ISIGN, K><<<grid, kBatchBlock, 0, st>>>(segs, off, nseg, active); break;
+    BK(KD_WILSON) BK(KA_WILSON) BK(KD_CLOVER) BK(KA_CLOVER) BK(KD_TM) BK(KA_TM) BK(KD_FULL)
+#undef BK
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// segs / off / active are device arrays; off has nseg + 1 entries
+template <int P>
+hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
+                                 const int *active, int grid, hipStream_t st) {
+  if (c12)
+    return isign > 0 ? launch_batch3<P, true, 1>(kind, segs, off, nseg, active, grid, st)
+                     : launch_batch3<P, true, -1>(kind, segs, off, nseg, active, grid, st);
+  if constexpr (P != P16) {
+    return isign > 0 ? launch_batch3<P, false, 1>(kind, segs, off, nseg, active, grid, st)
+                     : launch_batch3<P, false, -1>(kind, segs, off, nseg, active, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 }  // namespace qphix_amd
