@@ -17,6 +17,8 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -81,6 +83,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-block", action="store_true")
     ap.add_argument("--no-multi", action="store_true",
                     help="skip the 12-RHS launches (they share the headline kernel's symbol)")
     ap.add_argument("--hop-config", default=None,
@@ -165,8 +168,11 @@ def main():
 
     if not args.no_solve:
         extra["solve"] = solve_compare(q, g64, u64)
+    if not args.no_block:
+        extra["multi_rhs_32x32x32x64"] = block_compare(q)
     if not args.no_batch:
         extra["batch_tmclover"] = batch_compare(q)
+        extra["doublet_tmclover_operator"] = doublet_operator(q)
 
     result = {
         "metric": "clover Dslash GB/s (% HBM peak); solve s to 1e-10, mixed vs FP64",
@@ -213,9 +219,173 @@ def multi_rhs(q, D, g32, u32, inv32, nr, reps, per_ms, timer):
     gbs = bytes_multi / (ms_multi * 1e-3) / 1e9
     log(f"clover dslash f32 c12, {nr} rhs: {ms_multi:.3f} ms ({ms_multi / nr:.4f} ms per rhs), "
         f"{gbs:.1f} GB/s algorithmic")
-    return {"ms": round(ms_multi, 4), "ms_per_rhs": round(ms_multi / nr, 5),
-            "gbs_algorithmic": round(gbs, 1),
-            "speedup_per_rhs_vs_single": round(per_ms / (ms_multi / nr), 3)}
+    out = {"ms": round(ms_multi, 4), "ms_per_rhs": round(ms_multi / nr, 5),
+           "gbs_algorithmic": round(gbs, 1),
+           "speedup_per_rhs_vs_single": round(per_ms / (ms_multi / nr), 3)}
+    # the matrix-core kernel on 16 right-hand sides: each link multiplies the
+    # 3 x 32 block of projected half spinors as a real block-diagonal 12 x 12
+    # by 12 x 16 product (two sites per v_mfma_f32_16x16x4f32 triple)
+    nm = 16
+    psis = [q.gen_spinor(g32, 100 + r, 1) for r in range(nm)]
+    outs = [g32.spinor() for _ in range(nm)]
+    q.call("qphix_tune_rhs", 8, nm)
+    D.clov_dslash_multi(outs, psis, u32[0], inv32, 1, 0)
+    q.sync()
+    timer.start()
+    for _ in range(reps):
+        D.clov_dslash_multi(outs, psis, u32[0], inv32, 1, 0)
+    ms_m = timer.stop() / reps
+    q.call("qphix_tune_rhs", 0, 0)
+    issued = 3072.0 * vh * nm   # MFMA flops issued per launch (incl. the block-diagonal zeros)
+    useful = 1152.0 * vh * nm   # 8 hops x 2 spin components x 6 x 6 real multiply-adds
+    gbs_m = (8 * 12 + 72 + nm * 48) * 4 * vh / (ms_m * 1e-3) / 1e9
+    out["mfma_16rhs"] = {"ms": round(ms_m, 4), "ms_per_rhs": round(ms_m / nm, 5),
+                         "gbs_algorithmic": round(gbs_m, 1),
+                         "mfma_tflops_issued": round(issued / (ms_m * 1e-3) / 1e12, 2),
+                         "mfma_tflops_useful": round(useful / (ms_m * 1e-3) / 1e12, 2),
+                         "mfma_utilisation_of_fp32_peak": round(issued / (ms_m * 1e-3) / 157.3e12, 4)}
+    log(f"clover dslash f32 c12, {nm} rhs on matrix cores: {ms_m:.3f} ms ({ms_m / nm:.4f} ms per rhs), "
+        f"MFMA {issued / (ms_m * 1e-3) / 1e12:.2f} TFLOP/s issued")
+    return out
+
+
+def doublet_operator(q, reps=50):
+    """Non-degenerate twisted-mass clover doublet Schur operator on 16^3x32
+    (mu = 0.01, eps = 0.005, kappa = 0.1240, c_sw = 1): four Wilson hops, the
+    two-flavour inverse clover term and the two-flavour A chi - D psi, per
+    application, in FP64 and FP32."""
+    L = [16, 16, 16, 32]
+    mu, eps = 0.01, 0.005
+    mass = 1.0 / (2 * 0.1240) - 4.0
+    g64 = q.Geometry(L, q.F64, False)
+    u64 = q.gen_gauge(g64, 9, 0.4)
+    A = q.clover_build(g64, u64, CSW, mass, 1, q.CLOV_A)
+    up = q.clover_build(g64, u64, CSW, mass, 0, q.CLOV_ND_UP, mu, eps)
+    dn = q.clover_build(g64, u64, CSW, mass, 0, q.CLOV_ND_DN, mu, eps)
+    off = q.clover_build(g64, u64, CSW, mass, 0, q.CLOV_ND_OFF, mu, eps)
+    timer = q.Timer()
+    out = {"lattice": L, "mu": mu, "epsilon": eps}
+    for name, prec in (("f64", q.F64), ("f32", q.F32)):
+        if prec == q.F64:
+            g, u, fA, fup, fdn, foff = g64, u64, A, up, dn, off
+        else:
+            g = q.Geometry(L, prec, False)
+            u = q.convert_gauge(g, g64, u64)
+            fA = q.convert_clover(g, g64, A)
+            fup = q.convert_clover(g, g64, up, full=True)
+            fdn = q.convert_clover(g, g64, dn, full=True)
+            foff = q.convert_clover(g, g64, off)
+        M = q.Operator(g, q.OP_NDTMCLOVER, u, clov=[None, fA], fullclov=[[fup, fdn], None],
+                       clov_odiag=[foff, None], twisted_mass=mu, epsilon=eps, t_boundary=-1.0)
+        x = [q.gen_spinor(g, 11, 1), q.gen_spinor(g, 12, 1)]
+        y = [g.spinor(), g.spinor()]
+        for _ in range(5):
+            M(y, x, 1, 1)
+        timer.start()
+        for _ in range(reps):
+            M(y, x, 1, 1)
+        ms = timer.stop() / reps
+        out[name] = {"ms_per_application": round(ms, 4),
+                     "flavour_sites_per_s": round(2 * g.vh / (ms * 1e-3), 1)}
+        log(f"doublet tm-clover operator {name} 16^3x32: {ms:.4f} ms per application")
+    return out
+
+
+def block_compare(q):
+    """Batched right-hand sides on the 32^3x64 clover operator (kappa = 0.1240,
+    c_sw = 1): 12 point sources (spin x colour at one seeded site) and 16
+    Gaussian sources solved on M^+M in FP32 with FP64 reductions, by block CG
+    and by lockstep CG (one Krylov space per source), to 1e-6; then multishift
+    CG with 12 shifts sigma_k = 10^(-4 + 4k/11) in FP64 to 1e-8 per shift.
+    True residuals are recomputed with the FP64 operator."""
+    import ctypes as C
+    import random
+    L = [32, 32, 32, 64]
+    mass = 1.0 / (2 * 0.1240) - 4.0
+    g64 = q.Geometry(L, q.F64, False)
+    u64 = q.gen_gauge(g64, 2, 0.4)
+    A = q.clover_build(g64, u64, CSW, mass, 1, q.CLOV_A)
+    Ai = q.clover_build(g64, u64, CSW, mass, 0, q.CLOV_AINV)
+    M64 = q.Operator(g64, q.OP_CLOVER, u64, clov=[None, A], invclov=[Ai, None], t_boundary=-1.0)
+    g32 = q.Geometry(L, q.F32, True)
+    u32 = q.convert_gauge(g32, g64, u64)
+    A32, Ai32 = q.convert_clover(g32, g64, A), q.convert_clover(g32, g64, Ai)
+    M32 = q.Operator(g32, q.OP_CLOVER, u32, clov=[None, A32], invclov=[Ai32, None], t_boundary=-1.0)
+    vh = g32.vh
+    site = random.Random(12).randrange(vh)
+    e = np.zeros(vh * 24)
+    rhs64 = []
+    for k in range(12):
+        e[site * 24 + 2 * k] = 1.0
+        rhs64.append(g64.spinor().upload(e, 1))
+        e[site * 24 + 2 * k] = 0.0
+    rhs64 += [q.gen_spinor(g64, 500 + k, 1) for k in range(16)]
+    rhs32 = []
+    for b in rhs64:
+        f = g32.spinor()
+        q.call("qphix_convert_spinor", g32.h, f.p, C.c_double(1.0), g64.h, b.p)
+        rhs32.append(f)
+    n = len(rhs32)
+    tol = 1e-6
+
+    def true_resid(xs32):
+        worst = 0.0
+        x64, t1, t2 = g64.spinor(), g64.spinor(), g64.spinor()
+        for x, b in zip(xs32, rhs64):
+            q.call("qphix_convert_spinor", g64.h, x64.p, C.c_double(1.0), g32.h, x.p)
+            M64(t1, x64, 1, 1)
+            M64(t2, t1, -1, 1)
+            rn, bn = C.c_double(), C.c_double()
+            q.call("qphix_xmy_norm2", g64.h, t2.p, b.p, t2.p, C.byref(rn))
+            q.call("qphix_norm2_spinor", g64.h, b.p, C.byref(bn))
+            worst = max(worst, (rn.value / bn.value) ** 0.5)
+        return worst
+
+    out = {"lattice": L, "n_rhs": n, "tol": tol}
+    w = [g32.spinor() for _ in range(n)]
+    M32.apply_multi(w, rhs32, 1, 1)  # untimed: tunes the multi-RHS kernels
+    q.sync()
+    xb = [g32.spinor() for _ in range(n)]
+    t0 = time.perf_counter()
+    stb = q.InvBlockCG(M32, 3000).solve_multi(xb, rhs32, tol)
+    q.sync()
+    tb = time.perf_counter() - t0
+    out["block_cg"] = {"seconds": round(tb, 3), "block_iters": stb[0].n_iters,
+                       "status": max(st.status for st in stb), "worst_true_rel_resid": true_resid(xb)}
+    xl = [g32.spinor() for _ in range(n)]
+    t0 = time.perf_counter()
+    stl = q.InvCG(M32, 5000).solve_multi(xl, rhs32, tol)
+    q.sync()
+    tl = time.perf_counter() - t0
+    out["lockstep_cg"] = {"seconds": round(tl, 3), "max_iters": max(st.n_iters for st in stl),
+                          "status": max(st.status for st in stl), "worst_true_rel_resid": true_resid(xl)}
+    log(f"block cg 32^3x64 x{n}: {tb:.2f} s ({stb[0].n_iters} block iterations), lockstep cg "
+        f"{tl:.2f} s (max {out['lockstep_cg']['max_iters']} iterations)")
+    del xb, xl, w, rhs32
+    # multishift CG, one Gaussian source, FP64
+    shifts = [10 ** (-4 + 4 * k / 11) for k in range(12)]
+    b = rhs64[12]
+    xs = [g64.spinor() for _ in shifts]
+    ms = q.MInvCG(M64, 5000, len(shifts))
+    q.sync()
+    t0 = time.perf_counter()
+    nit, fin, mv = ms(xs, b, shifts, [1e-8] * len(shifts))
+    q.sync()
+    tm = time.perf_counter() - t0
+    t1, t2 = g64.spinor(), g64.spinor()
+    worst = 0.0
+    for sg, x in zip(shifts, xs):
+        M64(t1, x, 1, 1)
+        M64(t2, t1, -1, 1)
+        q.call("qphix_axpy", g64.h, C.c_double(sg), x.p, t2.p)
+        rn, bn = C.c_double(), C.c_double()
+        q.call("qphix_xmy_norm2", g64.h, t2.p, b.p, t2.p, C.byref(rn))
+        q.call("qphix_norm2_spinor", g64.h, b.p, C.byref(bn))
+        worst = max(worst, (rn.value / bn.value) ** 0.5)
+    out["multishift_cg"] = {"shifts": len(shifts), "seconds": round(tm, 3), "iters": nit,
+                            "mv_apps": mv, "worst_true_rel_resid_over_shifts": worst}
+    log(f"multishift cg 32^3x64, 12 shifts: {tm:.2f} s ({nit} iterations), worst true resid {worst:.2e}")
+    return out
 
 
 def solve_compare(q, g64, u64):
@@ -273,6 +443,24 @@ def solve_compare(q, g64, u64):
                                     "rel_err_vs_fp64": float((dn.value / xn.value) ** 0.5)}
     out["speedup"] = round(t64 / tm, 3)
     log(f"solve: fp64 {t64:.2f} s ({st.n_iters} it), mixed {tm:.2f} s, speedup {t64 / tm:.2f}")
+    # the clover Dslash A^-1 D psi of each precision the solve uses
+    timer = q.Timer()
+    per = {}
+    for name, g, u, inv, nb in (("f64_c18", g64, u64, inv64[0], BYTES_SITE[("f64", False)]),
+                                ("f16_c12", g16, u16, inv16[0], BYTES_SITE[("f16", True)])):
+        Dp = q.Dslash(g, t_boundary=-1.0)
+        ps, o = q.gen_spinor(g, 5, 1), g.spinor()
+        for _ in range(5):
+            Dp.clov_dslash(o, ps, u[0], inv, 1, 0)
+        timer.start()
+        for _ in range(20):
+            Dp.clov_dslash(o, ps, u[0], inv, 1, 0)
+        ms = timer.stop() / 20
+        gbs = nb * g.vh / (ms * 1e-3) / 1e9
+        per[name] = {"ms": round(ms, 4), "bytes_per_site": nb, "gbs_algorithmic": round(gbs, 1),
+                     "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"clover dslash {name}: {ms:.4f} ms, {gbs:.1f} GB/s algorithmic")
+    out["dslash_by_precision"] = per
     return out
 
 

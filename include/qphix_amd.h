@@ -70,6 +70,12 @@ int qphix_set_hop_block(int block);
 /* mode 0: no tuning; 1: autotune on first use; 2: force (block, bz, slabs):
  * bz z-planes per tile, slabs t-slabs (bz = 0: natural site order) */
 int qphix_tune_config(int mode, int block, int bz, int slabs);
+/* multi-RHS tiling: reg sources per work-item (1, 2 or 4; each link and
+ * clover block is loaded once for all of them; 8 selects the matrix-core
+ * kernel for FP32 fields, 8 sources per work-group, other precisions then
+ * fail with QPHIX_EHIP) and rt sources per launch (1..32, a multiple of reg);
+ * reg = 0 returns to autotuning. Clears the table of tuned configurations. */
+int qphix_tune_rhs(int reg, int rt);
 const char *qphix_tune_summary(void);
 
 /* ---- geometry: replaces Geometry<FT,V,S,c12>(latt, By, Bz, NCores, Sy, Sz,
@@ -263,9 +269,25 @@ int qphix_solver_destroy(qphix_solver *s);
 int qphix_solve(qphix_solver *s, void *const x[], const void *const rhs[], double rsd_target,
                 int isign, int verbose, int target_cb, int resid_type,
                 qphix_solve_stats *st);
-/* CG on M^+M for n right-hand sides in lockstep, one launch per step for all
- * unconverged right-hand sides (no reference counterpart; per right-hand side
- * the iteration is that of invcg.h:88-456); st[n] */
+/* Block BLAS over sets of spinor fields (no reference counterpart; used by the
+ * block solver). FP64 or FP32 fields, set sizes 1..32, synchronous.
+ * Gram: out[(i * nb + j) * 2 + {0,1}] = sum_sites conj(a_i) . b_j (FP64).
+ * Combine: o_j = y_j + sum_i a_i mat_ij, mat row-major na x nb complex
+ * (interleaved re, im); y may be NULL (no addition); o_j may alias a_i, y_j. */
+int qphix_block_gram(const qphix_geom *g, const void *const a[], int na, const void *const b[], int nb,
+                     double *out);
+int qphix_block_combine(const qphix_geom *g, const void *const a[], int na, const double *mat,
+                        const void *const y[], void *const o[], int nb);
+/* Block CG on M^+M (no reference counterpart): the n <= 32 right-hand sides of
+ * one qphix_solve_multi call share a block Krylov space; the residual block is
+ * kept orthonormal by a Cholesky QR each iteration. FP64 or FP32 fields,
+ * single flavour. n_iters is the number of block iterations, rsd_sq_final the
+ * relative recursive residual of each right-hand side. */
+int qphix_block_cg_create(qphix_op *M, int max_iters, qphix_solver **s);
+/* With a CG solver: CG on M^+M for n right-hand sides in lockstep, one launch
+ * per step for all unconverged right-hand sides (no reference counterpart; per
+ * right-hand side the iteration is that of invcg.h:88-456). With a block CG
+ * solver: the block iteration above. st[n] */
 int qphix_solve_multi(qphix_solver *s, void *const x[], const void *const rhs[], int n,
                       double rsd_target, int isign, int verbose, int target_cb, int resid_type,
                       qphix_solve_stats *st);

@@ -423,6 +423,13 @@ def InvCG(M, max_iters):
     return Solver(h, M)
 
 
+def InvBlockCG(M, max_iters):
+    """Block CG on M^+M for up to 32 right-hand sides (use solve_multi)."""
+    h = VP()
+    call("qphix_block_cg_create", M.h, max_iters, C.byref(h))
+    return Solver(h, M)
+
+
 def InvBiCGStab(M, max_iters):
     h = VP()
     call("qphix_bicgstab_create", M.h, max_iters, C.byref(h))
@@ -503,6 +510,22 @@ def norm2(geom, x):
     r = C.c_double()
     call("qphix_norm2_spinor", geom.h, x.p, C.byref(r))
     return r.value
+
+
+def block_gram(geom, a, b):
+    """G[i, j] = <a_i, b_j> (complex numpy matrix)."""
+    o = np.zeros(len(a) * len(b) * 2)
+    call("qphix_block_gram", geom.h, _pa(a), len(a), _pa(b), len(b), dptr(o))
+    return (o[0::2] + 1j * o[1::2]).reshape(len(a), len(b))
+
+
+def block_combine(geom, a, mat, y, o):
+    """o_j = y_j + sum_i a_i mat[i, j] (y None: no addition)."""
+    mat = np.asarray(mat, dtype=complex)
+    m = np.empty(mat.size * 2)
+    m[0::2], m[1::2] = mat.real.ravel(), mat.imag.ravel()
+    call("qphix_block_combine", geom.h, _pa(a), len(a), dptr(m), _pa(y) if y is not None else None,
+         _pa(o), len(o))
 
 
 def norm2_multi(geom, xs):

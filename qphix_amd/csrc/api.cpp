@@ -9,12 +9,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <complex>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/qphix_amd.h"
 #include "batch.hpp"
+#include "blockblas.hpp"
 #include "fields.hpp"
 #include "hop_impl.hpp"
 
@@ -54,7 +56,7 @@ struct qphix_op {
   void *tmp2[2];
   std::vector<void *> mtmp;  // per-right-hand-side temporaries of qphix_op_apply_multi
 };
-enum { S_CG = 0, S_BICGSTAB = 1, S_RICHARDSON = 2, S_MINVCG = 3 };
+enum { S_CG = 0, S_BICGSTAB = 1, S_RICHARDSON = 2, S_MINVCG = 3, S_BLOCKCG = 4 };
 struct qphix_solver {
   int kind;
   qphix_op *M;
@@ -584,11 +586,13 @@ extern "C" int qphix_dslash_destroy(qphix_dslash *d) {
 struct HopCfg {
   int block, bz, slabs;
   double ms;
-  int rt;  // right-hand sides per launch (multi-RHS keys)
+  int rt;   // right-hand sides per launch (multi-RHS keys)
+  int reg;  // right-hand sides per work-item (register tile; 1 = one per work-item)
 };
 static std::map<std::string, HopCfg> g_tuned;
 static int g_tune_enabled = -1;            // -1: read QPHIX_TUNE on first use
-static HopCfg g_forced = {0, 0, 0, 0, 0};  // block > 0: every hop uses this configuration
+static HopCfg g_forced = {0, 0, 0, 0, 0, 1};  // block > 0: every hop uses this configuration
+static int g_forced_reg = 0, g_forced_rt = 0;  // reg > 0: multi-RHS tiling set by qphix_tune_rhs
 
 static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block) {
   switch (g.prec) {
@@ -596,6 +600,46 @@ static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const H
     case P32: return launch_hop_prec<P32>(kind, g.c12 != 0, isign, a, block, g_stream);
     default: return launch_hop_prec<P16>(kind, g.c12 != 0, isign, a, block, g_stream);
   }
+}
+
+// launch of a.nrhs right-hand sides with `reg` of them per work-item; reg = 8
+// is the matrix-core kernel (FP32 storage), 8 right-hand sides per work-group
+static hipError_t launch_hop_tile_raw(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block, int reg) {
+  if (reg == 8) {
+    if (g.prec != P32) return hipErrorInvalidValue;
+    return launch_hop_mfma_f32(kind, g.c12 != 0, isign, a, g_stream);
+  }
+  switch (g.prec) {
+    case P64: return launch_hop_tile_prec<P64>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
+    case P32: return launch_hop_tile_prec<P32>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
+    default: return launch_hop_tile_prec<P16>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
+  }
+}
+
+// a.nrhs right-hand sides (pointer tables filled) in one or two launches:
+// multiples of `reg` through the register-tiled kernel, the rest one per work-item
+static hipError_t launch_hop_group(const GeomDesc &g, int kind, int isign, HopArgs a, int block, int reg) {
+  const int n = a.nrhs;
+  const int main = reg > 1 ? n - n % reg : 0;
+  if (main > 0) {
+    a.nrhs = main;
+    hipError_t e = launch_hop_tile_raw(g, kind, isign, a, block, reg);
+    if (e != hipSuccess) return e;
+  }
+  const int rest = n - main;
+  if (rest == 0) return hipSuccess;
+  a.nrhs = rest;
+  for (int r = 0; r < rest; r++) {
+    a.psis[r] = a.psis[main + r];
+    a.outs[r] = a.outs[main + r];
+    a.chis[r] = a.chis[main + r];
+  }
+  if (rest == 1) {
+    a.psi = a.psis[0];
+    a.out = a.outs[0];
+    a.chi = a.chis[0];
+  }
+  return launch_hop_raw(g, kind, isign, a, block);
 }
 
 static bool tuning_on(const GeomDesc &g) {
@@ -612,15 +656,18 @@ static size_t spinor_bytes(const GeomDesc &g) {
          : g.prec == P32 ? field_bytes<P32>(kSpinorReals, g.vs) : field_bytes<P16>(kSpinorReals, g.vs);
 }
 
-// time `reps` launches of a; returns ms per launch (1e30 if it cannot launch)
-static double time_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block, int reps) {
-  if (launch_hop_raw(g, kind, isign, a, block) != hipSuccess) {
+// time `reps` launches of a (reg > 0: the multi-RHS group launch); returns ms
+// per launch (1e30 if it cannot launch)
+static double time_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block, int reps, int reg = 0) {
+  auto go = [&]() { return reg > 0 ? launch_hop_group(g, kind, isign, a, block, reg)
+                                   : launch_hop_raw(g, kind, isign, a, block); };
+  if (go() != hipSuccess) {
     (void)hipGetLastError();
     return 1e30;
   }
   if (hipEventRecord(g_ev0, g_stream) != hipSuccess) return 1e30;
   for (int r = 0; r < reps; r++)
-    if (launch_hop_raw(g, kind, isign, a, block) != hipSuccess) return 1e30;
+    if (go() != hipSuccess) return 1e30;
   if (hipEventRecord(g_ev1, g_stream) != hipSuccess || hipEventSynchronize(g_ev1) != hipSuccess) return 1e30;
   float f = 0;
   if (hipEventElapsedTime(&f, g_ev0, g_ev1) != hipSuccess) return 1e30;
@@ -628,7 +675,7 @@ static double time_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a,
 }
 
 static void tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, HopCfg &best) {
-  best = HopCfg{g_hop_block, 0, 1, -1, 1};
+  best = HopCfg{g_hop_block, 0, 1, -1, 1, 1};
   if (g_forced.block > 0) {
     best = g_forced;
     if (best.bz > 0 && (g.L[2] % best.bz || g.L[3] % best.slabs)) {
@@ -640,11 +687,11 @@ static void tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, 
   if (!tuning_on(g)) return;
   std::vector<HopCfg> cand;
   for (int blk : {128, 256}) {
-    cand.push_back({blk, 0, 1, 0, 1});
+    cand.push_back({blk, 0, 1, 0, 1, 1});
     for (int bz : {2, 4, 6, 8, 12, 16, 24})
-      for (int sl : {8, 16})
+      for (int sl : {1, 2, 8, 16})
         if (bz <= g.L[2] && g.L[2] % bz == 0 && g.L[3] % sl == 0 && g.L[3] / sl >= 2)
-          cand.push_back({blk, bz, sl, 0, 1});
+          cand.push_back({blk, bz, sl, 0, 1, 1});
   }
   DevBuf scratch;
   if (hipMalloc(&scratch.p, spinor_bytes(g)) != hipSuccess) {
@@ -657,15 +704,29 @@ static void tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, 
   for (HopCfg &c : cand) {
     a.bz = c.bz;
     a.slabs = c.slabs;
-    c.ms = time_hop(g, kind, isign, a, c.block, 3);
+    c.ms = time_hop(g, kind, isign, a, c.block, 4);
     if (best.ms < 0 || c.ms < best.ms) best = c;
   }
 }
 
-// right-hand-side tile for n right-hand sides, given the single-RHS traversal
+// right-hand-side tiling for n right-hand sides, given the single-RHS
+// traversal: `reg` sources per work-item (register tile: each link and clover
+// block is loaded once for all of them) and `rt` sources per launch (the
+// work-groups of one site range for all rt sources run together and share the
+// remaining link and clover reads through the L2)
 static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs &a0, int n, HopCfg &cfg) {
   cfg.rt = n < kMaxRhs ? n : kMaxRhs;
-  if (g_forced.block > 0 || !tuning_on(g) || n == 1) return;
+  cfg.reg = 1;
+  if (g_forced_reg > 0) {
+    cfg.reg = g_forced_reg;
+    cfg.rt = g_forced_rt;
+    return;
+  }
+  if (g_forced.block > 0) {
+    cfg.reg = g_forced.reg;
+    return;
+  }
+  if (!tuning_on(g) || n == 1) return;
   const int rmax = cfg.rt;
   std::vector<DevBuf> outs(rmax);
   for (int r = 0; r < rmax; r++)
@@ -677,23 +738,28 @@ static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs 
   a.bz = cfg.bz;
   a.slabs = cfg.slabs;
   double best = -1;
-  for (int rt : {1, 2, 4, 8, 16, 32}) {
-    if (rt > rmax) break;
-    a.nrhs = rt;
-    for (int r = 0; r < rt; r++) {
-      a.psis[r] = a0.psis[r % n];
-      a.chis[r] = a0.chis[r % n];
-      a.outs[r] = outs[r].p;
-    }
-    if (rt == 1) {
-      a.psi = a.psis[0];
-      a.chi = a.chis[0];
-      a.out = a.outs[0];
-    }
-    const double per = time_hop(g, kind, isign, a, cfg.block, 2) / rt;
-    if (best < 0 || per < best) {
-      best = per;
-      cfg.rt = rt;
+  for (int reg : {1, 2, 4, 8}) {
+    if (reg == 8 && g.prec != P32) continue;
+    for (int rt : {1, 2, 4, 8, 16, 32}) {
+      if (rt > rmax || rt < reg || rt % reg) continue;
+      a.nrhs = rt;
+      for (int r = 0; r < rt; r++) {
+        a.psis[r] = a0.psis[r % n];
+        a.chis[r] = a0.chis[r % n];
+        a.outs[r] = outs[r].p;
+      }
+      if (rt == 1) {
+        a.psi = a.psis[0];
+        a.chi = a.chis[0];
+        a.out = a.outs[0];
+      }
+      const double per = time_hop(g, kind, isign, a, cfg.block, 2, reg) / rt;
+      if (best < 0 || per < best) {
+        best = per;
+        cfg.rt = rt;
+        cfg.reg = reg;
+        cfg.ms = per;
+      }
     }
   }
 }
@@ -780,21 +846,21 @@ static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void
   a.bz = cfg.bz;
   a.slabs = cfg.slabs;
   const int rt = n == 1 ? 1 : (cfg.rt > 0 ? cfg.rt : 1);
+  const int reg = n == 1 ? 1 : cfg.reg;
   for (int r0 = 0; r0 < n; r0 += rt) {
     const int m = (n - r0) < rt ? (n - r0) : rt;
     a.nrhs = m;
+    for (int r = 0; r < m; r++) {
+      a.psis[r] = psi[r0 + r];
+      a.outs[r] = res[r0 + r];
+      a.chis[r] = needs_chi ? chi[r0 + r] : nullptr;
+    }
     if (m == 1) {
       a.psi = psi[r0];
       a.out = res[r0];
       a.chi = needs_chi ? chi[r0] : nullptr;
-    } else {
-      for (int r = 0; r < m; r++) {
-        a.psis[r] = psi[r0 + r];
-        a.outs[r] = res[r0 + r];
-        a.chis[r] = needs_chi ? chi[r0 + r] : nullptr;
-      }
     }
-    hipError_t e = launch_hop_raw(g, kind, isign, a, cfg.block);
+    hipError_t e = launch_hop_group(g, kind, isign, a, cfg.block, reg);
     if (e != hipSuccess) return fail(QPHIX_EHIP, "hop kernel launch: %s", hipGetErrorString(e));
   }
   return QPHIX_OK;
@@ -817,7 +883,18 @@ extern "C" int qphix_tune_config(int mode, int block, int bz, int slabs) {
     return fail(QPHIX_EDOMAIN, "forced configuration block %d bz %d slabs %d", block, bz, slabs);
   g_tuned.clear();
   g_tune_enabled = mode == 1 ? 1 : 0;
-  g_forced = mode == 2 ? HopCfg{block, bz, bz > 0 ? slabs : 1, 0, kMaxRhs} : HopCfg{0, 0, 0, 0, 0};
+  g_forced = mode == 2 ? HopCfg{block, bz, bz > 0 ? slabs : 1, 0, kMaxRhs, 1} : HopCfg{0, 0, 0, 0, 0, 1};
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_tune_rhs(int reg, int rt) {
+  if (reg != 0 && reg != 1 && reg != 2 && reg != 4 && reg != 8)
+    return fail(QPHIX_EDOMAIN, "register tile %d (1, 2, 4, or 8 for the matrix-core kernel)", reg);
+  if (reg > 0 && (rt < reg || rt > kMaxRhs || rt % reg))
+    return fail(QPHIX_EDOMAIN, "RHS per launch %d with register tile %d", rt, reg);
+  g_tuned.clear();
+  g_forced_reg = reg;
+  g_forced_rt = reg > 0 ? rt : 0;
   return QPHIX_OK;
 }
 
@@ -827,8 +904,9 @@ extern "C" const char *qphix_tune_summary(void) {
   out.clear();
   for (auto &kv : g_tuned) {
     char line[256];
-    snprintf(line, sizeof(line), "%s: block %d bz %d slabs %d rhs-tile %d %.4f ms\n", kv.first.c_str(),
-             kv.second.block, kv.second.bz, kv.second.slabs, kv.second.rt, kv.second.ms);
+    snprintf(line, sizeof(line), "%s: block %d bz %d slabs %d rhs-tile %d reg-tile %d %.4f ms\n",
+             kv.first.c_str(), kv.second.block, kv.second.bz, kv.second.slabs, kv.second.rt, kv.second.reg,
+             kv.second.ms);
     out += line;
   }
   return out.c_str();
@@ -1120,6 +1198,31 @@ static int mf_cdot(const qphix_geom *g, int nf, const void *const *l, const void
   }
   return QPHIX_OK;
 }
+// <l, r> and |l|^2 in one sweep, summed over flavours
+static int mf_cdotnorm(const qphix_geom *g, int nf, const void *const *l, const void *const *r, double o[3]) {
+  o[0] = o[1] = o[2] = 0;
+  for (int f = 0; f < nf; f++) {
+    double t[3];
+    TRY(qphix_inner_product_norm(g, l[f], r[f], t));
+    for (int k = 0; k < 3; k++) o[k] += t[k];
+  }
+  return QPHIX_OK;
+}
+// BiCGStab x/r update that also returns <r0, r_new>: o = {|r|^2, Re, Im}
+static int mf_rxupdate_cdot(const qphix_geom *g, int nf, void *const *x, void *const *r, const void *const *t,
+                            const void *const *p, const void *const *r0, const double omega[2],
+                            const double alpha[2], double o[3]) {
+  o[0] = o[1] = o[2] = 0;
+  for (int f = 0; f < nf; f++) {
+    BlasArgs a = bargs();
+    a.x = t[f]; a.y = p[f]; a.z = r0[f]; a.o0 = x[f]; a.o1 = r[f];
+    a.a[0] = omega[0]; a.a[1] = omega[1]; a.a[2] = alpha[0]; a.a[3] = alpha[1];
+    double v[3];
+    TRY(run_blas(g, B_RXUPD_CDOT, a, v));
+    for (int k = 0; k < 3; k++) o[k] += v[k];
+  }
+  return QPHIX_OK;
+}
 static int op_apply_v(qphix_op *M, SV &res, const void *const *in, int isign, int cb) {
   return qphix_op_apply(M, res.data(), in, isign, cb);
 }
@@ -1301,8 +1404,13 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
   double rho_prev[2] = {1, 0}, alpha[2] = {1, 0}, omega[2] = {1, 0}, rho[2] = {0, 0};
   bool notconv = true;
   int k;
+  // <r0, r>: a separate sweep on the first iteration, afterwards a by-product
+  // of the previous iteration's x/r update
+  double r0r[2];
+  TRY(mf_cdot(g, nf, cv(r0), cv(r), r0r));
   for (k = 1; k <= s->max_iters && notconv; k++) {
-    TRY(mf_cdot(g, nf, cv(r0), cv(r), rho));
+    rho[0] = r0r[0];
+    rho[1] = r0r[1];
     if (rho[0] == 0 && rho[1] == 0) {
       st->rsd_sq_final = r_norm;
       st->n_iters = k;
@@ -1330,23 +1438,22 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
     for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_s_update(g, alpha, r[f], v[f]));
     TRY(op_apply_v(M, t, cv(r), isign, cb));
     st->mv_apps++;
-    double t_norm;
-    TRY(mf_norm2(g, nf, cv(t), t_norm));
+    double tr[3];  // <t, r> and |t|^2 in one sweep
+    TRY(mf_cdotnorm(g, nf, cv(t), cv(r), tr));
+    const double t_norm = tr[2];
     if (t_norm == 0) {
       st->rsd_sq_final = r_norm;
       st->n_iters = k;
       st->status = QPHIX_EBREAKDOWN;
       return QPHIX_OK;
     }
-    TRY(mf_cdot(g, nf, cv(t), cv(r), omega));
-    omega[0] /= t_norm;
-    omega[1] /= t_norm;
-    r_norm = 0;
-    for (int f = 0; f < nf; f++) {
-      double rn;
-      TRY(qphix_bicgstab_rxupdate(g, x[f], r[f], t[f], p[f], omega, alpha, &rn));
-      r_norm += rn;
-    }
+    omega[0] = tr[0] / t_norm;
+    omega[1] = tr[1] / t_norm;
+    double ru[3];
+    TRY(mf_rxupdate_cdot(g, nf, x, r.data(), cv(t), cv(p), cv(r0), omega, alpha, ru));
+    r_norm = ru[0];
+    r0r[0] = ru[1];
+    r0r[1] = ru[2];
     if (verbose) printf("BICGSTAB: iter %d r_norm = %e target = %e\n", k, r_norm, rsd_sq);
     if (r_norm < rsd_sq) notconv = false;
   }
@@ -1589,15 +1696,321 @@ static int solve_cg_multi(qphix_solver *s, void *const *x, const void *const *rh
   return QPHIX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Block CG on M^+M (no reference counterpart; the reference solves right-hand
+// sides one at a time with invcg.h). The n right-hand sides share one block
+// Krylov space. Form: block CG with the residual block kept orthonormal by a
+// Cholesky QR every iteration (Dubrulle's retooled block CG), which avoids the
+// ill-conditioned Gram matrices of the plain O'Leary recurrence:
+//   R = B - A X = Q C,  P = Q
+//   repeat: Z = A P;  xi = (P^H Z)^-1;  X += P xi C;  Q zeta = Q - Z xi (QR);
+//           P = Q + P zeta^H;  C = zeta C
+// so that R = Q C throughout and the residual norm of right-hand side j is the
+// norm of column j of C. The n x n algebra runs on the host in FP64; Gram
+// matrices are FP64 reductions on the device.
+// ---------------------------------------------------------------------------
+typedef std::complex<double> cd;
+struct CMat {  // n x n complex, row-major
+  int n;
+  std::vector<cd> v;
+  explicit CMat(int n_ = 0) : n(n_), v((size_t)n_ * n_) {}
+  cd &operator()(int i, int j) { return v[(size_t)i * n + j]; }
+  cd operator()(int i, int j) const { return v[(size_t)i * n + j]; }
+};
+static CMat cm_mul(const CMat &a, const CMat &b) {
+  CMat c(a.n);
+  for (int i = 0; i < a.n; i++)
+    for (int k = 0; k < a.n; k++) {
+      const cd x = a(i, k);
+      for (int j = 0; j < a.n; j++) c(i, j) += x * b(k, j);
+    }
+  return c;
+}
+static CMat cm_adj(const CMat &a) {
+  CMat c(a.n);
+  for (int i = 0; i < a.n; i++)
+    for (int j = 0; j < a.n; j++) c(i, j) = std::conj(a(j, i));
+  return c;
+}
+// upper-triangular U with G = U^H U (G Hermitian; its lower triangle is used
+// after symmetrisation); false if G is not numerically positive definite
+static bool cm_chol_upper(const CMat &g, CMat &u) {
+  const int n = g.n;
+  u = CMat(n);
+  for (int j = 0; j < n; j++) {
+    double d = g(j, j).real();
+    for (int k = 0; k < j; k++) d -= std::norm(u(k, j));
+    if (!(d > 0)) return false;
+    const double r = sqrt(d);
+    u(j, j) = r;
+    for (int i = j + 1; i < n; i++) {
+      cd x = 0.5 * (g(j, i) + std::conj(g(i, j)));
+      for (int k = 0; k < j; k++) x -= std::conj(u(k, j)) * u(k, i);
+      u(j, i) = x / r;
+    }
+  }
+  return true;
+}
+static CMat cm_inv_upper(const CMat &u) {
+  const int n = u.n;
+  CMat x(n);
+  for (int j = 0; j < n; j++) {
+    x(j, j) = 1.0 / u(j, j);
+    for (int i = j - 1; i >= 0; i--) {
+      cd s = 0;
+      for (int k = i + 1; k <= j; k++) s += u(i, k) * x(k, j);
+      x(i, j) = -s / u(i, i);
+    }
+  }
+  return x;
+}
+
+static double *g_gram_part = nullptr;  // device, kGramPartials doubles
+static double *g_gram_dev = nullptr;   // device, one Gram matrix
+static double *g_gram_host = nullptr;  // pinned host, one Gram matrix
+static double *g_mat_dev = nullptr;    // device, 4 matrix slots
+static double *g_mat_host = nullptr;   // pinned host, 4 matrix slots
+static int g_mat_slot = 0;
+constexpr size_t kMatDoubles = (size_t)kMaxRhs * kMaxRhs * 2;
+
+static int block_buffers() {
+  if (g_gram_part) return QPHIX_OK;
+  HIPCHK(hipMalloc(&g_gram_part, kGramPartials * sizeof(double)));
+  HIPCHK(hipMalloc(&g_gram_dev, kMatDoubles * sizeof(double)));
+  HIPCHK(hipHostMalloc(&g_gram_host, kMatDoubles * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipMalloc(&g_mat_dev, 4 * kMatDoubles * sizeof(double)));
+  HIPCHK(hipHostMalloc(&g_mat_host, 4 * kMatDoubles * sizeof(double), hipHostMallocDefault));
+  return QPHIX_OK;
+}
+
+// G = A^H B over the fields a[0..n), b[0..n) (synchronises)
+static int block_gram(const qphix_geom *g, void *const *a, void *const *b, int n, CMat &G) {
+  BlockArgs ba;
+  memset(&ba, 0, sizeof(ba));
+  for (int i = 0; i < n; i++) {
+    ba.a[i] = a[i];
+    ba.b[i] = b[i];
+  }
+  ba.na = ba.nb = n;
+  HIPCHK(block_gram_launch(g->d, ba, g_gram_part, g_gram_dev, g_stream));
+  HIPCHK(hipMemcpyAsync(g_gram_host, g_gram_dev, (size_t)n * n * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  G = CMat(n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) G(i, j) = cd(g_gram_host[(i * n + j) * 2], g_gram_host[(i * n + j) * 2 + 1]);
+  return QPHIX_OK;
+}
+
+// o_j = y_j + sum_i a_i M_ij (y null: no addition). Matrix slots cycle over
+// four pinned staging buffers; every solver step synchronises (a Gram matrix)
+// before a slot is reused.
+static int block_combine(const qphix_geom *g, void *const *a, const CMat &M, void *const *y, void *const *o) {
+  const int n = M.n;
+  double *h = g_mat_host + g_mat_slot * kMatDoubles;
+  double *d = g_mat_dev + g_mat_slot * kMatDoubles;
+  g_mat_slot = (g_mat_slot + 1) & 3;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      h[(i * n + j) * 2] = M(i, j).real();
+      h[(i * n + j) * 2 + 1] = M(i, j).imag();
+    }
+  HIPCHK(hipMemcpyAsync(d, h, (size_t)n * n * 2 * sizeof(double), hipMemcpyHostToDevice, g_stream));
+  BlockArgs ba;
+  memset(&ba, 0, sizeof(ba));
+  for (int i = 0; i < n; i++) {
+    ba.a[i] = a[i];
+    ba.o[i] = o[i];
+    ba.y[i] = y ? y[i] : nullptr;
+  }
+  ba.na = ba.nb = n;
+  HIPCHK(block_combine_launch(g->d, ba, d, g_stream));
+  return QPHIX_OK;
+}
+
+// block BLAS entry points (host matrices, synchronous Gram)
+extern "C" int qphix_block_gram(const qphix_geom *g, const void *const a[], int na, const void *const b[], int nb,
+                                double *out) {
+  BLAS_PRE(); CHKPTR(a); CHKPTR(b); CHKPTR(out);
+  if (na < 1 || nb < 1 || na > kMaxRhs || nb > kMaxRhs) return fail(QPHIX_EDOMAIN, "block sizes %d x %d", na, nb);
+  if (g->d.prec == P16) return fail(QPHIX_EDOMAIN, "block BLAS needs FP64 or FP32 fields");
+  TRY(block_buffers());
+  BlockArgs ba;
+  memset(&ba, 0, sizeof(ba));
+  for (int i = 0; i < na; i++) {
+    if (!a[i]) return fail(QPHIX_EINVAL, "null field");
+    ba.a[i] = a[i];
+  }
+  for (int j = 0; j < nb; j++) {
+    if (!b[j]) return fail(QPHIX_EINVAL, "null field");
+    ba.b[j] = b[j];
+  }
+  ba.na = na;
+  ba.nb = nb;
+  HIPCHK(block_gram_launch(g->d, ba, g_gram_part, g_gram_dev, g_stream));
+  HIPCHK(hipMemcpyAsync(g_gram_host, g_gram_dev, (size_t)na * nb * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  memcpy(out, g_gram_host, (size_t)na * nb * 2 * sizeof(double));
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_block_combine(const qphix_geom *g, const void *const a[], int na, const double *mat,
+                                   const void *const y[], void *const o[], int nb) {
+  BLAS_PRE(); CHKPTR(a); CHKPTR(mat); CHKPTR(o);
+  if (na < 1 || nb < 1 || na > kMaxRhs || nb > kMaxRhs) return fail(QPHIX_EDOMAIN, "block sizes %d x %d", na, nb);
+  if (g->d.prec == P16) return fail(QPHIX_EDOMAIN, "block BLAS needs FP64 or FP32 fields");
+  TRY(block_buffers());
+  BlockArgs ba;
+  memset(&ba, 0, sizeof(ba));
+  for (int i = 0; i < na; i++) {
+    if (!a[i]) return fail(QPHIX_EINVAL, "null field");
+    ba.a[i] = a[i];
+  }
+  for (int j = 0; j < nb; j++) {
+    if (!o[j]) return fail(QPHIX_EINVAL, "null field");
+    ba.o[j] = o[j];
+    ba.y[j] = y ? y[j] : nullptr;
+  }
+  ba.na = na;
+  ba.nb = nb;
+  // synchronous staging of the matrix (the call may reuse mat at once)
+  HIPCHK(hipStreamSynchronize(g_stream));
+  memcpy(g_mat_host, mat, (size_t)na * nb * 2 * sizeof(double));
+  HIPCHK(hipMemcpyAsync(g_mat_dev, g_mat_host, (size_t)na * nb * 2 * sizeof(double), hipMemcpyHostToDevice,
+                        g_stream));
+  HIPCHK(block_combine_launch(g->d, ba, g_mat_dev, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+
+static int solve_block_cg(qphix_solver *s, void *const *x, const void *const *rhs, int n, double rsd_target,
+                          int isign, int verbose, int cb, int rtype, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  TRY(block_buffers());
+  while ((int)s->mwork.size() < 4 * n) {
+    void *p = nullptr;
+    TRY(qphix_alloc_spinor(g, &p));
+    s->mwork.push_back(p);
+  }
+  void **mp = &s->mwork[0], **zz = &s->mwork[n], **pp = &s->mwork[2 * n], **qq = &s->mwork[3 * n];
+  for (int r = 0; r < n; r++) memset(&st[r], 0, sizeof(st[r]));
+  std::vector<double> chi_sq(n), rsd_sq(n), tmp(n);
+  {
+    MultiBlasArgs m = margs(n);
+    for (int r = 0; r < n; r++) m.x[r] = rhs[r];
+    TRY(run_blas_multi(g, B_NORM2, m, chi_sq.data()));
+  }
+  for (int r = 0; r < n; r++) {
+    if (chi_sq[r] == 0) return fail(QPHIX_EDOMAIN, "right-hand side %d is zero", r);
+    rsd_sq[r] = rsd_target * rsd_target * (rtype == QPHIX_RESID_RELATIVE ? chi_sq[r] : 1.0);
+  }
+  auto mdagm = [&](void *const *in, void **out) -> int {
+    TRY(qphix_op_apply_multi(M, mp, (const void *const *)in, n, isign, cb));
+    TRY(qphix_op_apply_multi(M, out, (const void *const *)mp, n, -isign, cb));
+    for (int r = 0; r < n; r++) st[r].mv_apps += 2;
+    return QPHIX_OK;
+  };
+  // R = B - A X  (into qq)
+  TRY(mdagm(x, zz));
+  {
+    MultiBlasArgs m = margs(n);
+    for (int r = 0; r < n; r++) {
+      m.o0[r] = qq[r];
+      m.x[r] = rhs[r];
+      m.y[r] = zz[r];
+    }
+    TRY(run_blas_multi(g, B_XMYNORM2, m, tmp.data()));
+  }
+  bool all_done = true;
+  for (int r = 0; r < n; r++) all_done = all_done && tmp[r] <= rsd_sq[r];
+  if (all_done) {
+    for (int r = 0; r < n; r++) st[r].rsd_sq_final = tmp[r] / chi_sq[r];
+    return QPHIX_OK;
+  }
+  // R = Q C
+  CMat G, C, U;
+  TRY(block_gram(g, qq, qq, n, G));
+  if (!cm_chol_upper(G, C)) return fail(QPHIX_EBREAKDOWN, "block CG: residual block is rank deficient");
+  TRY(block_combine(g, qq, cm_inv_upper(C), nullptr, qq));
+  {
+    MultiBlasArgs m = margs(n);
+    for (int r = 0; r < n; r++) {
+      m.o0[r] = pp[r];
+      m.x[r] = qq[r];
+    }
+    TRY(run_blas_multi(g, B_COPY, m, nullptr));
+  }
+  int k;
+  bool conv = false, broke = false;
+  std::vector<double> rn(n);
+  for (k = 1; k <= s->max_iters; k++) {
+    TRY(mdagm(pp, zz));
+    TRY(block_gram(g, pp, zz, n, G));  // P^H A P
+    CMat L;
+    if (!cm_chol_upper(G, L)) {
+      broke = true;
+      break;
+    }
+    const CMat Li = cm_inv_upper(L);
+    const CMat xi = cm_mul(Li, cm_adj(Li));  // (P^H A P)^-1 = L^-1 L^-H
+    TRY(block_combine(g, pp, cm_mul(xi, C), x, x));
+    CMat mxi = xi;
+    for (cd &e : mxi.v) e = -e;
+    TRY(block_combine(g, zz, mxi, qq, qq));  // W = Q - Z xi
+    TRY(block_gram(g, qq, qq, n, G));
+    CMat zeta;
+    if (!cm_chol_upper(G, zeta)) {
+      broke = true;
+      break;
+    }
+    TRY(block_combine(g, qq, cm_inv_upper(zeta), nullptr, qq));  // Q = W zeta^-1
+    TRY(block_combine(g, pp, cm_adj(zeta), qq, pp));              // P = Q + P zeta^H
+    C = cm_mul(zeta, C);
+    conv = true;
+    for (int j = 0; j < n; j++) {
+      double c2 = 0;
+      for (int i = 0; i < n; i++) c2 += std::norm(C(i, j));
+      rn[j] = c2;
+      conv = conv && c2 <= rsd_sq[j];
+    }
+    if (verbose) {
+      double worst = 0;
+      for (int j = 0; j < n; j++) worst = fmax(worst, rn[j] / rsd_sq[j]);
+      printf("BLOCK CG: iter %d worst r^2 / target %e\n", k, worst);
+    }
+    if (conv) break;
+  }
+  for (int r = 0; r < n; r++) {
+    st[r].n_iters = k;
+    st[r].rsd_sq_final = rn[r] / chi_sq[r];
+    st[r].status = broke ? QPHIX_EBREAKDOWN : (conv ? QPHIX_OK : QPHIX_ENOCONV);
+  }
+  return QPHIX_OK;
+}
+
+extern "C" int qphix_block_cg_create(qphix_op *M, int max_iters, qphix_solver **s) {
+  TRY(qphix_cg_create(M, max_iters, s));
+  (*s)->kind = S_BLOCKCG;
+  return QPHIX_OK;
+}
+
 extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *const rhs[], int n, double rsd_target,
                                  int isign, int verbose, int target_cb, int resid_type, qphix_solve_stats *st) {
   ENSURE_INIT();
   CHKPTR(s); CHKPTR(x); CHKPTR(rhs); CHKPTR(st);
   TRY(check_cb(target_cb));
   if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1");
-  if (s->kind != S_CG) return fail(QPHIX_EINVAL, "multi-right-hand-side solves use a CG solver");
+  if (s->kind != S_CG && s->kind != S_BLOCKCG)
+    return fail(QPHIX_EINVAL, "multi-right-hand-side solves use a CG or block CG solver");
   if (s->M->nflav != 1) return fail(QPHIX_EDOMAIN, "multi-right-hand-side CG is for single-flavour operators");
   if (n < 1) return fail(QPHIX_EDOMAIN, "number of right-hand sides %d", n);
+  if (s->kind == S_BLOCKCG) {
+    if (n > kMaxRhs) return fail(QPHIX_EDOMAIN, "block CG takes at most %d right-hand sides, got %d", kMaxRhs, n);
+    if (s->M->g->d.prec == P16) return fail(QPHIX_EDOMAIN, "block CG needs FP64 or FP32 fields");
+    return solve_block_cg(s, x, rhs, n, rsd_target, isign, verbose, target_cb, resid_type, st);
+  }
   for (int r0 = 0; r0 < n; r0 += kMaxRhs) {
     const int m = (n - r0) < kMaxRhs ? (n - r0) : kMaxRhs;
     TRY(solve_cg_multi(s, x + r0, rhs + r0, m, rsd_target, isign, verbose, target_cb, resid_type, st + r0));

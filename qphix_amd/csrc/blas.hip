@@ -19,7 +19,7 @@ __host__ __device__ constexpr int nred_of(int op) {
   return (op == B_NORM2 || op == B_AXPYNORM2 || op == B_XMYNORM2 || op == B_XMY2NORM2 || op == B_RMAMMP ||
           op == B_RICH || op == B_RXUPD)
              ? 1
-             : (op == B_CDOT) ? 2 : (op == B_CDOTNORM) ? 3 : 0;
+             : (op == B_CDOT) ? 2 : (op == B_CDOTNORM || op == B_RXUPD_CDOT) ? 3 : 0;
 }
 int blas_nred(int op) { return nred_of(op); }
 
@@ -141,7 +141,9 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
       y[k + 1] = y[k + 1] + sg * pi;
     }
     store_spinor<P, true>(a.o0, vs, s, y);
-  } else if constexpr (OP == B_RXUPD) {  // x += omega r + alpha p ; r -= omega t  (x=t, y=p, o0=x, o1=r)
+  } else if constexpr (OP == B_RXUPD || OP == B_RXUPD_CDOT) {
+    // x += omega r + alpha p ; r -= omega t  (x=t, y=p, o0=x, o1=r); the _CDOT
+    // form also returns <r0, r_new> (z = r0) for the next BiCGStab iteration
     T r[N], xx[N];
     const T wr = a0, wi = a1, ar = (T)a.a[2], ai = (T)a.a[3];
     load_spinor<P, true>(a.o1, vs, s, r);
@@ -162,6 +164,14 @@ __device__ inline void blas_site(const BlasArgs &a, size_t vs, long s, double *r
     }
     red[0] += sq(r);
     store_spinor<P, true>(a.o1, vs, s, r);
+    if constexpr (OP == B_RXUPD_CDOT) {
+      load_spinor<P, true>(a.z, vs, s, y);
+#pragma unroll
+      for (int k = 0; k < N; k += 2) {
+        red[1] += (double)y[k] * (double)r[k] + (double)y[k + 1] * (double)r[k + 1];
+        red[2] += (double)y[k] * (double)r[k + 1] - (double)y[k + 1] * (double)r[k];
+      }
+    }
   } else if constexpr (OP == B_MRUPD) {  // x += a r ; r -= a Mr   (x = Mr, o0 = x, o1 = r)
     T r[N], xx[N];
     load_spinor<P, true>(a.o1, vs, s, r);
@@ -310,7 +320,7 @@ static hipError_t launch_prec(const GeomDesc &g, int op, const BlasArgs &a, doub
     OPCASE(B_COPY) OPCASE(B_ZERO) OPCASE(B_AXY) OPCASE(B_AX) OPCASE(B_YPEQX) OPCASE(B_YMEQX)
     OPCASE(B_AYPX) OPCASE(B_AXPY) OPCASE(B_AXPBY) OPCASE(B_NORM2) OPCASE(B_AXPYNORM2) OPCASE(B_XMYNORM2)
     OPCASE(B_XMY2NORM2) OPCASE(B_RMAMMP) OPCASE(B_RICH) OPCASE(B_BICG_XMY) OPCASE(B_CDOT) OPCASE(B_CDOTNORM)
-    OPCASE(B_PUPD) OPCASE(B_SUPD) OPCASE(B_CAXPY) OPCASE(B_RXUPD) OPCASE(B_MRUPD) OPCASE(B_TWM)
+    OPCASE(B_PUPD) OPCASE(B_SUPD) OPCASE(B_CAXPY) OPCASE(B_RXUPD) OPCASE(B_RXUPD_CDOT) OPCASE(B_MRUPD) OPCASE(B_TWM)
     OPCASE(B_TWM2) OPCASE(B_CLOVMULT) OPCASE(B_FULLMULT) OPCASE(B_TWOFLAV_INVCLOV)
     default: return hipErrorInvalidValue;
   }

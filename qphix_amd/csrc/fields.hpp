@@ -39,7 +39,7 @@ enum BlasOp {
   B_COPY = 0, B_ZERO, B_AXY, B_AX, B_YPEQX, B_YMEQX, B_AYPX, B_AXPY, B_AXPBY,
   B_NORM2, B_AXPYNORM2, B_XMYNORM2, B_XMY2NORM2, B_RMAMMP, B_RICH, B_BICG_XMY,
   B_CDOT, B_CDOTNORM, B_PUPD, B_SUPD, B_CAXPY, B_RXUPD, B_MRUPD, B_TWM, B_TWM2,
-  B_CLOVMULT, B_FULLMULT, B_TWOFLAV_INVCLOV, NUM_BLAS_OPS
+  B_CLOVMULT, B_FULLMULT, B_TWOFLAV_INVCLOV, B_RXUPD_CDOT, NUM_BLAS_OPS
 };
 struct BlasArgs {
   const void *x, *y, *z, *w;  // read-only operands

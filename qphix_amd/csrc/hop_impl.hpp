@@ -97,9 +97,12 @@ template <int P, bool C12> struct GaugeLayout {
   static constexpr int pair_words = 2 * link_reals / PT<P>::per_word;
 };
 
+// The twisted-boundary phase (tr, ti) is applied unconditionally, (1, 0) when
+// the dimension has none: a branch here would split the kernel into basic
+// blocks and stop the scheduler from issuing the next hop's loads early.
 template <int MU, int S, bool FWD, bool C12, typename T>
-__device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3], T coef, bool tbc,
-                                 T tr, T ti, cx<T> (&acc)[4][3]) {
+__device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3], T coef, T tr, T ti,
+                                 cx<T> (&acc)[4][3]) {
   cx<T> h[2][3], uh[2][3];
   project<MU, S>(psi, h);
 #pragma unroll
@@ -118,13 +121,11 @@ __device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3
       }
       uh[r][i] = sum;
     }
-  if (tbc) {
-    const cx<T> ph = mk(tr, FWD ? ti : -ti);
+  const cx<T> ph = mk(tr, FWD ? ti : -ti);
 #pragma unroll
-    for (int r = 0; r < 2; r++)
+  for (int r = 0; r < 2; r++)
 #pragma unroll
-      for (int i = 0; i < 3; i++) uh[r][i] = ph * uh[r][i];
-  }
+    for (int i = 0; i < 3; i++) uh[r][i] = ph * uh[r][i];
   recon_acc<MU, S>(acc, uh, coef);
 }
 
@@ -134,7 +135,7 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
   typedef typename PT<P>::calc T;
   typedef GaugeLayout<P, C12> GL;
   const bool tbc = (a.tbc_mask >> MU) & 1;
-  const T tr = (T)a.tbc_r[MU], ti = (T)a.tbc_i[MU];
+  const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
   T sp[kSpinorReals];
   cx<T> psi[4][3], U[3][3];
   if constexpr (GL::per_link) {
@@ -144,24 +145,24 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
     load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
-    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
+    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tr, ti, acc);
     // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
     load_spinor<P>(psi_p, a.vs, sf, sp);
     load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
-    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tbc, tr, ti, acc);
+    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr, ti, acc);
   } else {
     T lv[2 * GL::link_reals];
     load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
     load_spinor<P>(psi_p, a.vs, sb, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
-    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tbc, tr, ti, acc);
+    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tr, ti, acc);
     load_spinor<P>(psi_p, a.vs, sf, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv + GL::link_reals, U);
-    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tbc, tr, ti, acc);
+    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr, ti, acc);
   }
 }
 
@@ -247,123 +248,281 @@ template <typename T> __device__ inline void add_ig5(T g, const cx<T> (&in)[4][3
   }
 }
 
-// the hop for traversal index k (< a.vh) of one right-hand side
-template <int P, bool C12, int ISIGN, int KIND>
-__device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, void *out_p, const void *chi_p) {
-  typedef typename PT<P>::calc T;
-  long s = k;
+// neighbour indices and temporal coefficients of one target site
+struct SiteNbrs {
+  unsigned s, xb, xf, yb, yf, zb, zf, tb, tf;
+  bool t_lo, t_hi;  // t == 0, t == Nt-1
+};
+
+// target site of traversal index k and its eight neighbours. 32-bit index
+// arithmetic throughout: a parity of one GPU's lattice has fewer than 2^31
+// sites, and 64-bit division would cost a branchy library sequence.
+__device__ inline SiteNbrs site_nbrs(const HopArgs &a, unsigned k) {
+  SiteNbrs n;
+  unsigned s = k;
   if (a.bz > 0) {
-    const int plane = a.nxh * a.ny;
-    const int nts = a.nt / a.slabs, nzb = a.nz / a.bz;
-    long q = k / plane;
-    const int p = (int)(k - q * plane);
-    const int zi = (int)(q % a.bz);
-    q /= a.bz;
-    const int ti = (int)(q % nts);
+    const unsigned plane = (unsigned)(a.nxh * a.ny);
+    const unsigned nts = (unsigned)(a.nt / a.slabs), bz = (unsigned)a.bz, nzb = (unsigned)(a.nz / a.bz);
+    unsigned q = s / plane;
+    const unsigned p = s - q * plane;
+    const unsigned zi = q % bz;
+    q /= bz;
+    const unsigned ti = q % nts;
     q /= nts;
-    const int zb = (int)(q % nzb);
-    const int slab = (int)(q / nzb);
-    s = ((long)(slab * nts + ti) * a.nz + zb * a.bz + zi) * plane + p;
+    const unsigned zb = q % nzb;
+    const unsigned slab = q / nzb;
+    s = ((slab * nts + ti) * (unsigned)a.nz + zb * bz + zi) * plane + p;
   }
+  const unsigned nxh = (unsigned)a.nxh, ny = (unsigned)a.ny, nz = (unsigned)a.nz, nt = (unsigned)a.nt;
+  unsigned r = s;
+  const unsigned xh = r % nxh;
+  r /= nxh;
+  const unsigned y = r % ny;
+  r /= ny;
+  const unsigned z = r % nz;
+  const unsigned t = r / nz;
+  const unsigned nx = 2 * nxh;
+  const unsigned x = 2 * xh + ((y + z + t + (unsigned)a.cb) & 1);
+  const unsigned row = s - xh;
+  const unsigned sy = nxh, sz = nxh * ny, st = sz * nz;
+  n.s = s;
+  n.xb = row + ((x == 0 ? nx - 1 : x - 1) >> 1);
+  n.xf = row + ((x == nx - 1 ? 0 : x + 1) >> 1);
+  n.yb = y == 0 ? s + (ny - 1) * sy : s - sy;
+  n.yf = y == ny - 1 ? s - (ny - 1) * sy : s + sy;
+  n.zb = z == 0 ? s + (nz - 1) * sz : s - sz;
+  n.zf = z == nz - 1 ? s - (nz - 1) * sz : s + sz;
+  n.tb = t == 0 ? s + (nt - 1) * st : s - st;
+  n.tf = t == nt - 1 ? s - (nt - 1) * st : s + st;
+  n.t_lo = t == 0;
+  n.t_hi = t == nt - 1;
+  return n;
+}
 
-  // coordinates of the target site
-  int r = (int)s;
-  const int xh = r % a.nxh;
-  r /= a.nxh;
-  const int y = r % a.ny;
-  r /= a.ny;
-  const int z = r % a.nz;
-  const int t = r / a.nz;
-  const int nx = 2 * a.nxh;
-  const int x = 2 * xh + ((y + z + t + a.cb) & 1);
+// One direction mu of the hop for R right-hand sides: each link is loaded and
+// rebuilt once and multiplies the projected neighbours of all R sources (the
+// 3 x (2 R) block contraction of a multi-RHS Dslash).
+template <int P, bool C12, int ISIGN, int MU, bool FWD, int R>
+__device__ inline void hop_half_r(const cx<typename PT<P>::calc> (&U)[3][3], const HopArgs &a,
+                                  const void *const (&psi)[R], unsigned sn, typename PT<P>::calc coef,
+                                  typename PT<P>::calc tr, typename PT<P>::calc ti,
+                                  cx<typename PT<P>::calc> (&acc)[R][4][3]) {
+  typedef typename PT<P>::calc T;
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    T sp[kSpinorReals];
+    cx<T> ps[4][3];
+    load_spinor<P>(psi[j], a.vs, sn, sp);
+    to_spinor(sp, ps);
+    apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12>(U, ps, coef, tr, ti, acc[j]);
+  }
+}
 
-  const long row = s - xh;
-  const long xb = row + ((x == 0 ? nx - 1 : x - 1) >> 1);
-  const long xf = row + ((x == nx - 1 ? 0 : x + 1) >> 1);
-  const long sy = a.nxh, sz = (long)a.nxh * a.ny, st = sz * a.nz;
-  const long yb = y == 0 ? s + (a.ny - 1) * sy : s - sy;
-  const long yf = y == a.ny - 1 ? s - (a.ny - 1) * sy : s + sy;
-  const long zb = z == 0 ? s + (a.nz - 1) * sz : s - sz;
-  const long zf = z == a.nz - 1 ? s - (a.nz - 1) * sz : s + sz;
-  const long tb = t == 0 ? s + (a.nt - 1) * st : s - st;
-  const long tf = t == a.nt - 1 ? s - (a.nt - 1) * st : s + st;
+template <int P, bool C12, int ISIGN, int MU, int R>
+__device__ inline void hop_dim_r(const HopArgs &a, const void *const (&psi)[R], unsigned s, unsigned sb, unsigned sf,
+                                 typename PT<P>::calc cb_coef, typename PT<P>::calc cf_coef,
+                                 cx<typename PT<P>::calc> (&acc)[R][4][3]) {
+  typedef typename PT<P>::calc T;
+  typedef GaugeLayout<P, C12> GL;
+  const bool tbc = (a.tbc_mask >> MU) & 1;
+  const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
+  cx<T> U[3][3];
+  if constexpr (GL::per_link) {
+    T lv[GL::link_reals];
+    // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
+    load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv, U);
+    hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, acc);
+    // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
+    load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv, U);
+    hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, acc);
+  } else {
+    T lv[2 * GL::link_reals];
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv, U);
+    hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, acc);
+    make_link<C12>(lv + GL::link_reals, U);
+    hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, acc);
+  }
+}
 
-  const T cs = (T)a.cs;
-  const T ctb = (T)(t == 0 ? a.ctb : a.ct);
-  const T ctf = (T)(t == a.nt - 1 ? a.ctf : a.ct);
+// packed Hermitian clover block B applied to R spinors (block loaded once)
+template <int P, int B, int R, typename T>
+__device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s, const cx<T> (&in)[R][4][3],
+                                            cx<T> (&out)[R][4][3]) {
+  T m[36];
+  load_clover_block<P, B>(clov, vs, s, m);
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    cx<T> v[6], w[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
+    herm6(m, v, w);
+#pragma unroll
+    for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = w[i];
+  }
+}
 
-  cx<T> acc[4][3];
+template <int P, int B, int R, typename T>
+__device__ inline void full_block_apply_r(const void *fcl, size_t vs, long s, const cx<T> (&in)[R][4][3],
+                                          cx<T> (&out)[R][4][3]) {
+  constexpr int nw = words_of<P>(kFullCloverReals);
+  constexpr int bw = nw / 2;
+  T m[72];
+  load_range<P, nw, B * bw, bw, true>(fcl, vs, s, m);
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    cx<T> v[6], w[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
+    full6(m, v, w);
+#pragma unroll
+    for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = w[i];
+  }
+}
+
+// accumulator start of the kernel family: 0, or the A chi part of the
+// achimbdpsi forms, for R right-hand sides at site s
+template <int P, int ISIGN, int KIND, int R>
+__device__ inline void hop_init_r(const HopArgs &a, unsigned s, const void *const (&chi)[R],
+                                  cx<typename PT<P>::calc> (&acc)[R][4][3]) {
+  typedef typename PT<P>::calc T;
   if constexpr (KIND == KA_WILSON || KIND == KA_CLOVER || KIND == KA_TM) {
-    T cv[kSpinorReals];
-    cx<T> chi[4][3];
-    load_spinor<P, true>(chi_p, a.vs, s, cv);
-    to_spinor(cv, chi);
+    cx<T> ch[R][4][3];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      T cv[kSpinorReals];
+      load_spinor<P, true>(chi[j], a.vs, s, cv);
+      to_spinor(cv, ch[j]);
+    }
     if constexpr (KIND == KA_WILSON) {
       const T al = (T)a.alpha;
 #pragma unroll
-      for (int sp = 0; sp < 4; sp++)
+      for (int j = 0; j < R; j++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) acc[sp][c] = scale(al, chi[sp][c]);
+        for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[j][sp][c] = scale(al, ch[j][sp][c]);
     } else if constexpr (KIND == KA_CLOVER) {
-      clover_block_apply<P, 0>(a.clov, a.vs, s, chi, acc);
-      clover_block_apply<P, 1>(a.clov, a.vs, s, chi, acc);
-      add_ig5((T)(ISIGN * a.rho), chi, acc);
+      clover_block_apply_r<P, 0, R>(a.clov, a.vs, s, ch, acc);
+      clover_block_apply_r<P, 1, R>(a.clov, a.vs, s, ch, acc);
+#pragma unroll
+      for (int j = 0; j < R; j++) add_ig5((T)(ISIGN * a.rho), ch[j], acc[j]);
     } else {
       const T al = (T)a.alpha;
 #pragma unroll
-      for (int sp = 0; sp < 4; sp++)
+      for (int j = 0; j < R; j++) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) acc[sp][c] = chi[sp][c];
-      add_ig5((T)(ISIGN * a.mu), chi, acc);
+        for (int sp = 0; sp < 4; sp++)
 #pragma unroll
-      for (int sp = 0; sp < 4; sp++)
+          for (int c = 0; c < 3; c++) acc[j][sp][c] = ch[j][sp][c];
+        add_ig5((T)(ISIGN * a.mu), ch[j], acc[j]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) acc[sp][c] = scale(al, acc[sp][c]);
+        for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[j][sp][c] = scale(al, acc[j][sp][c]);
+      }
     }
   } else {
 #pragma unroll
-    for (int sp = 0; sp < 4; sp++)
+    for (int j = 0; j < R; j++)
 #pragma unroll
-      for (int c = 0; c < 3; c++) acc[sp][c] = mk((T)0, (T)0);
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[j][sp][c] = mk((T)0, (T)0);
   }
-
-  hop_dim<P, C12, ISIGN, 0>(a, psi_p, s, xb, xf, cs, cs, acc);
-  hop_dim<P, C12, ISIGN, 1>(a, psi_p, s, yb, yf, cs, cs, acc);
-  hop_dim<P, C12, ISIGN, 2>(a, psi_p, s, zb, zf, cs, cs, acc);
-  hop_dim<P, C12, ISIGN, 3>(a, psi_p, s, tb, tf, ctb, ctf, acc);
-
-  T ov[kSpinorReals];
-  if constexpr (KIND == KD_CLOVER) {
-    cx<T> o[4][3];
-    clover_block_apply<P, 0>(a.clov, a.vs, s, acc, o);
-    clover_block_apply<P, 1>(a.clov, a.vs, s, acc, o);
-    from_spinor(o, ov);
-  } else if constexpr (KIND == KD_FULL) {
-    cx<T> o[4][3];
-    full_block_apply<P, 0>(a.clov, a.vs, s, acc, o);
-    full_block_apply<P, 1>(a.clov, a.vs, s, acc, o);
-    from_spinor(o, ov);
-  } else if constexpr (KIND == KD_TM) {
-    cx<T> o[4][3];
-#pragma unroll
-    for (int sp = 0; sp < 4; sp++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) o[sp][c] = acc[sp][c];
-    add_ig5((T)(-ISIGN * a.mu), acc, o);
-    const T mi = (T)a.muinv;
-#pragma unroll
-    for (int sp = 0; sp < 4; sp++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) o[sp][c] = scale(mi, o[sp][c]);
-    from_spinor(o, ov);
-  } else {
-    from_spinor(acc, ov);
-  }
-  store_spinor<P, true>(out_p, a.vs, s, ov);
 }
 
+// epilogue of the kernel family (A^-1, twisted-mass factors, full block) and
+// the store, for R right-hand sides at site s
+template <int P, int ISIGN, int KIND, int R>
+__device__ inline void hop_finish_r(const HopArgs &a, unsigned s, const cx<typename PT<P>::calc> (&acc)[R][4][3],
+                                    void *const (&out)[R]) {
+  typedef typename PT<P>::calc T;
+  if constexpr (KIND == KD_CLOVER || KIND == KD_FULL) {
+    cx<T> o[R][4][3];
+    if constexpr (KIND == KD_CLOVER) {
+      clover_block_apply_r<P, 0, R>(a.clov, a.vs, s, acc, o);
+      clover_block_apply_r<P, 1, R>(a.clov, a.vs, s, acc, o);
+    } else {
+      full_block_apply_r<P, 0, R>(a.clov, a.vs, s, acc, o);
+      full_block_apply_r<P, 1, R>(a.clov, a.vs, s, acc, o);
+    }
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      T ov[kSpinorReals];
+      from_spinor(o[j], ov);
+      store_spinor<P, true>(out[j], a.vs, s, ov);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      T ov[kSpinorReals];
+      if constexpr (KIND == KD_TM) {
+        cx<T> o[4][3];
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) o[sp][c] = acc[j][sp][c];
+        add_ig5((T)(-ISIGN * a.mu), acc[j], o);
+        const T mi = (T)a.muinv;
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) o[sp][c] = scale(mi, o[sp][c]);
+        from_spinor(o, ov);
+      } else {
+        from_spinor(acc[j], ov);
+      }
+      store_spinor<P, true>(out[j], a.vs, s, ov);
+    }
+  }
+}
+
+// the hop for traversal index k (< a.vh) of R right-hand sides psi[j] -> out[j]
+template <int P, bool C12, int ISIGN, int KIND, int R>
+__device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *const (&psi)[R], void *const (&out)[R],
+                                  const void *const (&chi)[R]) {
+  typedef typename PT<P>::calc T;
+  const SiteNbrs n = site_nbrs(a, k);
+  const unsigned s = n.s;
+  const T cs = (T)a.cs;
+  const T ctb = (T)(n.t_lo ? a.ctb : a.ct);
+  const T ctf = (T)(n.t_hi ? a.ctf : a.ct);
+
+  cx<T> acc[R][4][3];
+  hop_init_r<P, ISIGN, KIND, R>(a, s, chi, acc);
+  if constexpr (R == 1) {
+    hop_dim<P, C12, ISIGN, 0>(a, psi[0], s, n.xb, n.xf, cs, cs, acc[0]);
+    hop_dim<P, C12, ISIGN, 1>(a, psi[0], s, n.yb, n.yf, cs, cs, acc[0]);
+    hop_dim<P, C12, ISIGN, 2>(a, psi[0], s, n.zb, n.zf, cs, cs, acc[0]);
+    hop_dim<P, C12, ISIGN, 3>(a, psi[0], s, n.tb, n.tf, ctb, ctf, acc[0]);
+  } else {
+    hop_dim_r<P, C12, ISIGN, 0, R>(a, psi, s, n.xb, n.xf, cs, cs, acc);
+    hop_dim_r<P, C12, ISIGN, 1, R>(a, psi, s, n.yb, n.yf, cs, cs, acc);
+    hop_dim_r<P, C12, ISIGN, 2, R>(a, psi, s, n.zb, n.zf, cs, cs, acc);
+    hop_dim_r<P, C12, ISIGN, 3, R>(a, psi, s, n.tb, n.tf, ctb, ctf, acc);
+  }
+  hop_finish_r<P, ISIGN, KIND, R>(a, s, acc, out);
+}
+
+// the hop for traversal index k (< a.vh) of one right-hand side
 template <int P, bool C12, int ISIGN, int KIND>
-__global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
+__device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, void *out_p, const void *chi_p) {
+  const void *const psi[1] = {psi_p};
+  void *const out[1] = {out_p};
+  const void *const chi[1] = {chi_p};
+  hop_body_r<P, C12, ISIGN, KIND, 1>(a, (unsigned)k, psi, out, chi);
+}
+
+// minimum waves per SIMD the register allocation must allow for the hop kernels
+#ifndef QPHIX_HOP_WAVES
+#define QPHIX_HOP_WAVES 1
+#endif
+
+template <int P, bool C12, int ISIGN, int KIND>
+__global__ void __launch_bounds__(256, QPHIX_HOP_WAVES) hop_kernel(const HopArgs a) {
   int lb = xcd_block();
   const void *psi_p = a.psi, *chi_p = a.chi;
   void *out_p = a.out;
@@ -382,6 +541,32 @@ __global__ void __launch_bounds__(256) hop_kernel(const HopArgs a) {
   const long k = (long)lb * blockDim.x + threadIdx.x;
   if (k >= a.vh) return;
   hop_body<P, C12, ISIGN, KIND>(a, k, psi_p, out_p, chi_p);
+}
+
+// Multi-RHS hop with a register tile of R right-hand sides per work-item:
+// a.nrhs (a multiple of R) sources in a.psis; the RHS group is the fastest
+// index of the work-group order, so the groups sharing a site range run
+// together and also share links and clover through the L2.
+template <int P, bool C12, int ISIGN, int KIND, int R>
+__global__ void __launch_bounds__(256) hop_tile_kernel(const HopArgs a) {
+  int lb = xcd_block();
+  const int ng = a.nrhs / R;
+  const int g = lb % ng;
+  lb /= ng;
+  const unsigned k = (unsigned)lb * blockDim.x + threadIdx.x;
+  if (k >= (unsigned)a.vh) return;
+  // per-RHS pointers read in place from the kernel-argument segment (see hop_kernel)
+  const HopArgs *ka = (const HopArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  const void *psi[R];
+  void *out[R];
+  const void *chi[R];
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    psi[j] = ka->psis[g * R + j];
+    out[j] = ka->outs[g * R + j];
+    chi[j] = ka->chis[g * R + j];
+  }
+  hop_body_r<P, C12, ISIGN, KIND, R>(a, k, psi, out, chi);
 }
 
 // Batched hop: one launch covers several systems (possibly on different
@@ -410,5 +595,9 @@ __global__ void __launch_bounds__(256) hop_batch_kernel(const HopSeg *segs, cons
 
 template <int P>
 hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int block, hipStream_t st);
+template <int P>
+hipError_t launch_hop_tile_prec(int kind, bool c12, int isign, int rt, const HopArgs &a, int block, hipStream_t st);
+// matrix-core multi-RHS launch (FP32 storage, a.nrhs a multiple of 8)
+hipError_t launch_hop_mfma_f32(int kind, bool c12, int isign, const HopArgs &a, hipStream_t st);
 
 }  // namespace qphix_amd

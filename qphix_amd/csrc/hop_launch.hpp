@@ -32,6 +32,38 @@ hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int 
   return hipErrorInvalidValue;
 }
 
+// register-tiled multi-RHS launch: R right-hand sides per work-item, a.nrhs
+// a multiple of R
+template <int P, bool C12, int ISIGN, int R>
+static hipError_t launch_tile3(int kind, const HopArgs &a, int block, hipStream_t st) {
+  const dim3 grid((unsigned)(((a.vh + block - 1) / block) * (a.nrhs / R)));
+  switch (kind) {
+#define TK(K) \
+  case K: hop_tile_kernel<P, C12, ISIGN, K, R><<<grid, block, 0, st>>>(a); break;
+    TK(KD_WILSON) TK(KA_WILSON) TK(KD_CLOVER) TK(KA_CLOVER) TK(KD_TM) TK(KA_TM) TK(KD_FULL)
+#undef TK
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int P, bool C12>
+static hipError_t launch_tile2(int kind, int isign, int rt, const HopArgs &a, int block, hipStream_t st) {
+  if (rt == 2)
+    return isign > 0 ? launch_tile3<P, C12, 1, 2>(kind, a, block, st) : launch_tile3<P, C12, -1, 2>(kind, a, block, st);
+  if (rt == 4)
+    return isign > 0 ? launch_tile3<P, C12, 1, 4>(kind, a, block, st) : launch_tile3<P, C12, -1, 4>(kind, a, block, st);
+  return hipErrorInvalidValue;
+}
+
+template <int P>
+hipError_t launch_hop_tile_prec(int kind, bool c12, int isign, int rt, const HopArgs &a, int block, hipStream_t st) {
+  if (block <= 0 || block > 256 || (block & 63) || a.nrhs % rt) return hipErrorInvalidValue;
+  if (c12) return launch_tile2<P, true>(kind, isign, rt, a, block, st);
+  if constexpr (P != P16) return launch_tile2<P, false>(kind, isign, rt, a, block, st);
+  return hipErrorInvalidValue;
+}
+
 constexpr int kBatchBlock = 256;
 
 template <int P, bool C12, int ISIGN>

@@ -95,3 +95,71 @@ def test_multi_operator_and_cg():
     norms = q.norm2_multi(g, rhs)
     for r in range(n):
         assert abs(norms[r] - rhs_h[r] @ rhs_h[r]) <= 1e-13 * norms[r]
+
+
+@pytest.fixture
+def forced_rhs_tiling():
+    yield
+    q.call("qphix_tune_rhs", 0, 0)
+
+
+@pytest.mark.parametrize("reg,rt", [(1, 4), (2, 2), (2, 6), (4, 4), (4, 8), (8, 8), (8, 16)])
+@pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True)], ids=["f64", "f32c12", "f16c12"])
+def test_register_tiled_multi_rhs(forced_rhs_tiling, reg, rt, prec, c12):
+    """Each forced tiling (register tile reg, rt sources per launch; n = 5 or,
+    for the matrix-core kernel, 11, so that a remainder goes through the
+    one-per-work-item kernel) gives every right-hand side its single-RHS
+    result: exactly for the vector kernels (same expressions, same rounding),
+    to FP32 rounding for the matrix-core kernel (FP32 fields only; its dot
+    products are summed in the matrix unit's order), for the Wilson, clover
+    and twisted-mass-clover members of the kernel family."""
+    L = [4, 4, 4, 8]
+    mfma = reg == 8
+    n = 11 if mfma else 5
+    if mfma and prec != q.F32:
+        q.call("qphix_tune_rhs", reg, rt)
+        g = q.Geometry(L, prec, c12)
+        D = q.Dslash(g, t_boundary=-1.0)
+        ud = q.pack_gauge(g, oracle.gen_gauge(L, 33, 0.4))
+        ps = [g.spinor() for _ in range(n)]
+        with pytest.raises(q.QphixError):
+            D.dslash_multi([g.spinor() for _ in range(n)], ps, ud[0], 1, 0)
+        return
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    u = oracle.gen_gauge(L, 33, 0.4)
+    clov = oracle.clover_build(L, u, 1.0, 0.05)
+    g = q.Geometry(L, prec, c12)
+    ud = q.pack_gauge(g, u)
+    inv = g.clover().upload(oracle.clover_invert(clov[:vh * 72]))
+    A = g.clover().upload(clov[vh * 72:])
+    f = g.full_clover().upload(oracle.full_clover_tm_inverse(clov[:vh * 72], 0.03))
+    q.call("qphix_tune_rhs", reg, rt)
+    D = q.Dslash(g, t_boundary=-1.0, prec_mass_rho=0.02)
+    psi = [g.spinor().upload(oracle.gen_spinor(L, 70 + r, 1), 1) for r in range(n)]
+    chi = [g.spinor().upload(oracle.gen_spinor(L, 80 + r, 0), 0) for r in range(n)]
+
+    def check(multi, single, cb):
+        outs = [g.spinor() for _ in range(n)]
+        multi(outs)
+        for r in range(n):
+            one = g.spinor()
+            single(one, r)
+            got, exp = outs[r].download(cb), one.download(cb)
+            if mfma:
+                assert np.linalg.norm(got - exp) <= 2e-6 * np.linalg.norm(exp), (r, reg, rt)
+            else:
+                assert np.array_equal(got, exp), (r, reg, rt)
+
+    for isign in (1, -1):
+        check(lambda o: D.clov_dslash_multi(o, psi, ud[0], inv, isign, 0),
+              lambda o, r: D.clov_dslash(o, psi[r], ud[0], inv, isign, 0), 0)
+        check(lambda o: D.clov_achimbdpsi_multi(o, psi, chi, ud[0], A, 0.25, isign, 0),
+              lambda o, r: D.clov_achimbdpsi(o, psi[r], chi[r], ud[0], A, 0.25, isign, 0), 0)
+        check(lambda o: D.dslash_multi(o, psi, ud[0], isign, 0),
+              lambda o, r: D.dslash(o, psi[r], ud[0], isign, 0), 0)
+    # twisted-mass clover Schur operator (full-block inverse kernel, A chi + i rho g5 chi)
+    fm = g.full_clover().upload(oracle.full_clover_tm_inverse(clov[:vh * 72], -0.03))
+    M = q.Operator(g, q.OP_TMCLOVER, ud, clov=[None, A], fullclov=[[f, fm], None], twisted_mass=0.03,
+                   prec_mass_rho=0.03, t_boundary=-1.0)
+    for isign in (1, -1):
+        check(lambda o: M.apply_multi(o, psi, isign, 1), lambda o, r: M(o, psi[r], isign, 1), 1)
