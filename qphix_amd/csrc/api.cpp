@@ -70,6 +70,10 @@ struct qphix_solver {
   int max_shifts;
   // multi-right-hand-side CG work spinors (grown on demand)
   std::vector<void *> mwork;
+  // set by a caller that passes x = 0 (the defect-correction loop's inner
+  // solve): the initial residual is then the right-hand side itself and the
+  // operator is not applied to the zero vector
+  int x_zero = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -892,7 +896,8 @@ extern "C" int qphix_tune_rhs(int reg, int rt) {
     return fail(QPHIX_EDOMAIN, "register tile %d (1, 2, 4, or 8 for the matrix-core kernel)", reg);
   if (reg > 0 && (rt < reg || rt > kMaxRhs || rt % reg))
     return fail(QPHIX_EDOMAIN, "RHS per launch %d with register tile %d", rt, reg);
-  g_tuned.clear();
+  for (auto it = g_tuned.begin(); it != g_tuned.end();)  // multi-RHS entries only
+    it = it->first.find(" n") != std::string::npos ? g_tuned.erase(it) : std::next(it);
   g_forced_reg = reg;
   g_forced_rt = reg > 0 ? rt : 0;
   return QPHIX_OK;
@@ -1308,14 +1313,19 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
   TRY(mf_norm2(g, nf, rhs, chi_sq));
   double rsd_sq = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= chi_sq;
-  TRY(op_apply_v(M, mp, x, isign, cb));
-  TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
-  st->mv_apps += 2;
   double cp = 0;
-  for (int f = 0; f < nf; f++) {
-    double t;
-    TRY(qphix_xmy_norm2(g, r[f], rhs[f], mmp[f], &t));
-    cp += t;
+  if (s->x_zero) {
+    for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r[f], rhs[f]));
+    cp = chi_sq;
+  } else {
+    TRY(op_apply_v(M, mp, x, isign, cb));
+    TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+    st->mv_apps += 2;
+    for (int f = 0; f < nf; f++) {
+      double t;
+      TRY(qphix_xmy_norm2(g, r[f], rhs[f], mmp[f], &t));
+      cp += t;
+    }
   }
   if (verbose) printf("CG: ||r||^2 = %e target = %e\n", cp, rsd_sq);
   if (cp <= rsd_sq) {
@@ -1386,11 +1396,16 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
   TRY(mf_norm2(g, nf, rhs, rhs_sq));
   double rsd_sq = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= rhs_sq;
-  TRY(op_apply_v(M, r0, x, isign, cb));
-  st->mv_apps++;
-  for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r0[f]));
   double r_norm;
-  TRY(mf_norm2(g, nf, cv(r0), r_norm));
+  if (s->x_zero) {
+    for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r0[f], rhs[f]));
+    r_norm = rhs_sq;
+  } else {
+    TRY(op_apply_v(M, r0, x, isign, cb));
+    st->mv_apps++;
+    for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r0[f]));
+    TRY(mf_norm2(g, nf, cv(r0), r_norm));
+  }
   if (r_norm < rsd_sq) {
     st->rsd_sq_final = r_norm;
     st->n_iters = 0;
@@ -1528,7 +1543,10 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
       TRY(qphix_zero_spinor(gi, dxi[f]));
     }
     qphix_solve_stats ist;
-    TRY(qphix_solve(s->inner, dxi.data(), cv(ri), delta_use, isign, 0, cb, QPHIX_RESID_RELATIVE, &ist));
+    s->inner->x_zero = 1;  // dxi was just zeroed
+    const int ie = qphix_solve(s->inner, dxi.data(), cv(ri), delta_use, isign, 0, cb, QPHIX_RESID_RELATIVE, &ist);
+    s->inner->x_zero = 0;
+    TRY(ie);
     mv_inner += ist.mv_apps;
     fl_inner += ist.site_flops;
     for (int f = 0; f < nf; f++) TRY(qphix_convert_spinor(g, dx[f], r_norm, gi, dxi[f]));

@@ -516,13 +516,16 @@ __device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, voi
   hop_body_r<P, C12, ISIGN, KIND, 1>(a, (unsigned)k, psi, out, chi);
 }
 
-// minimum waves per SIMD the register allocation must allow for the hop kernels
-#ifndef QPHIX_HOP_WAVES
-#define QPHIX_HOP_WAVES 1
+// minimum waves per SIMD the register allocation must allow for the hop
+// kernels: the FP16 forms would otherwise take ~170 VGPRs (two waves per SIMD);
+// 168 gives three without spilling
+#ifndef QPHIX_HOP_WAVES16
+#define QPHIX_HOP_WAVES16 3
 #endif
+constexpr int hop_min_waves(int p) { return p == P16 ? QPHIX_HOP_WAVES16 : 1; }
 
 template <int P, bool C12, int ISIGN, int KIND>
-__global__ void __launch_bounds__(256, QPHIX_HOP_WAVES) hop_kernel(const HopArgs a) {
+__global__ void __launch_bounds__(256, hop_min_waves(P)) hop_kernel(const HopArgs a) {
   int lb = xcd_block();
   const void *psi_p = a.psi, *chi_p = a.chi;
   void *out_p = a.out;
