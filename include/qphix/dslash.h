@@ -69,6 +69,11 @@ class Dslash : public detail::DslashBase<FT, veclen, soalen, compress12> {
     detail::check(qphix_wilson_achimbdpsi(this->d_, res, psi, chi, u, alpha, beta, isign, cb),
                   "Dslash::dslashAChiMinusBDPsi");
   }
+  // dslash_def.h:54-58: one term of the hop, isign +1
+  void dslashDir(FourSpinorBlock *res, const FourSpinorBlock *psi, const SU3MatrixBlock *u, int cb, int dir)
+  {
+    detail::check(qphix_wilson_dslash_dir(this->d_, res, psi, u, cb, dir), "Dslash::dslashDir");
+  }
 };
 
 template <typename FT, int veclen, int soalen, bool compress12>

@@ -12,6 +12,7 @@
 #pragma once
 
 #include "dslash.h"
+#include "full_spinor.h"
 
 namespace QPhiX {
 
@@ -22,9 +23,31 @@ class EvenOddLinearOperator {
   static constexpr int num_flav = 1;
   virtual ~EvenOddLinearOperator() {}
   virtual void operator()(FourSpinorBlock *res, const FourSpinorBlock *in, int isign, int target_cb = 1) const = 0;
+  // multigrid-facing pieces (linearOp.h:26-35): the library operators provide
+  // them for Wilson and clover, the others report "not yet implemented" by a
+  // std::runtime_error as the reference prints it
+  virtual void M_offdiag(FourSpinorBlock *res, FourSpinorBlock const *in, int isign, int target_cb) const
+  {
+    void *r[1] = {res};
+    const void *i[1] = {in};
+    detail::check(qphix_op_m_offdiag(need_native(), r, i, isign, target_cb), "M_offdiag");
+  }
+  virtual void M_diag_inv(FourSpinorBlock *res, FourSpinorBlock const *in, int isign, int cb = 0) const
+  {
+    void *r[1] = {res};
+    const void *i[1] = {in};
+    detail::check(qphix_op_m_diag_inv(need_native(), r, i, isign, cb), "M_diag_inv");
+  }
   virtual Geometry<FT, veclen, soalen, compress12> &getGeometry() = 0;
   // library operator backing this object, or nullptr for user-defined operators
   virtual qphix_op *native() const { return nullptr; }
+
+ private:
+  qphix_op *need_native() const
+  {
+    if (!native()) throw std::runtime_error("operator has no library implementation of this piece");
+    return native();
+  }
 };
 
 template <typename FT, int veclen, int soalen, bool compress12>
@@ -149,7 +172,31 @@ class EvenOddCloverOperator : public EvenOddLinearOperator<FT, veclen, soalen, c
     n_.apply(r, i, isign, target_cb);
   }
 
+  // clover.h:196-251: A on both (or one) checkerboards, the unpreconditioned
+  // operator A - 1/2 D, and one direction of the hop (isign +1)
+  using SpinorFull = FullSpinor<FT, veclen, soalen, compress12>;
+  void M_diag(SpinorFull &res, const SpinorFull &in, int isign) const { diag(res, in, isign, -1); }
+  void M_diag(SpinorFull &res, const SpinorFull &in, int isign, int cb) const { diag(res, in, isign, cb); }
+  void M_unprec(SpinorFull &res, const SpinorFull &in, int isign) const
+  {
+    void *r[2] = {res.getCBData(0), res.getCBData(1)};
+    const void *i[2] = {in.getCBData(0), in.getCBData(1)};
+    detail::check(qphix_op_m_unprec(n_.op_, r, i, isign), "M_unprec");
+  }
+  void DslashDir(SpinorFull &res, const SpinorFull &in, int dir) const
+  {
+    void *r[2] = {res.getCBData(0), res.getCBData(1)};
+    const void *i[2] = {in.getCBData(0), in.getCBData(1)};
+    detail::check(qphix_op_dslash_dir(n_.op_, r, i, dir), "DslashDir");
+  }
+
  private:
+  void diag(SpinorFull &res, const SpinorFull &in, int isign, int cb) const
+  {
+    void *r[2] = {res.getCBData(0), res.getCBData(1)};
+    const void *i[2] = {in.getCBData(0), in.getCBData(1)};
+    detail::check(qphix_op_m_diag(n_.op_, r, i, isign, cb), "M_diag");
+  }
   void set(SU3MatrixBlock *u[2], CloverBlock *c0, CloverBlock *c1, CloverBlock *i0, CloverBlock *i1)
   {
     n_.p_.u[0] = u[0];

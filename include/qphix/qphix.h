@@ -4,4 +4,6 @@
 #include "dslash.h"
 #include "linear_ops.h"
 #include "solvers.h"
+#include "full_spinor.h"
+#include "unprec_solver_wrapper.h"
 #include "blas_new_c.h"

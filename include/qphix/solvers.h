@@ -227,6 +227,41 @@ class InvBiCGStab : public detail::NativeSolverBase<FT, veclen, soalen, compress
   }
 };
 
+// InvMR / InvMRSmoother (invmr.h:249-348)
+template <typename FT, int V, int S, bool compress12,
+          typename EvenOddLinearOperatorBase = EvenOddLinearOperator<FT, V, S, compress12>>
+class InvMR : public detail::NativeSolverBase<FT, V, S, compress12, EvenOddLinearOperatorBase::num_flav,
+                                              EvenOddLinearOperatorBase> {
+  typedef detail::NativeSolverBase<FT, V, S, compress12, EvenOddLinearOperatorBase::num_flav,
+                                   EvenOddLinearOperatorBase>
+      B;
+
+ public:
+  static constexpr int num_flav = EvenOddLinearOperatorBase::num_flav;
+  InvMR(EvenOddLinearOperatorBase &M, int MaxIters, double Omega) : B(M)
+  {
+    if (!M.native()) throw std::runtime_error("InvMR needs a library operator");
+    detail::check(qphix_mr_create(M.native(), MaxIters, Omega, 0, &this->s_), "InvMR");
+  }
+};
+
+template <typename FT, int V, int S, bool compress12,
+          typename EvenOddLinearOperatorBase = EvenOddLinearOperator<FT, V, S, compress12>>
+class InvMRSmoother : public detail::NativeSolverBase<FT, V, S, compress12, EvenOddLinearOperatorBase::num_flav,
+                                                      EvenOddLinearOperatorBase> {
+  typedef detail::NativeSolverBase<FT, V, S, compress12, EvenOddLinearOperatorBase::num_flav,
+                                   EvenOddLinearOperatorBase>
+      B;
+
+ public:
+  static constexpr int num_flav = EvenOddLinearOperatorBase::num_flav;
+  InvMRSmoother(EvenOddLinearOperatorBase &M, int MaxIters, double Omega) : B(M)
+  {
+    if (!M.native()) throw std::runtime_error("InvMRSmoother needs a library operator");
+    detail::check(qphix_mr_create(M.native(), MaxIters, Omega, 1, &this->s_), "InvMRSmoother");
+  }
+};
+
 template <typename FT, int V, int S, bool Compress, typename FTInner, int VInner, int SInner, bool CompressInner,
           bool MdagM = false, typename EvenOddLinearOperatorBase = EvenOddLinearOperator<FT, V, S, Compress>>
 class InvRichardsonMultiPrec

@@ -159,6 +159,11 @@ int qphix_wilson_dslash(qphix_dslash *d, void *res, const void *psi, const void 
                         int isign, int cb);
 int qphix_wilson_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi,
                             const void *u, double alpha, double beta, int isign, int cb);
+/* Dslash::dslashDir (dslash_def.h:54-58, dslash_body.h:1074-1137): one term of
+ * the hop with isign = +1; dir = 2 mu takes the forward neighbour psi(x + mu),
+ * dir = 2 mu + 1 the backward one */
+int qphix_wilson_dslash_dir(qphix_dslash *d, void *res, const void *psi, const void *u, int cb,
+                            int dir);
 /* ClovDslash::dslash, ::dslashAChiMinusBDPsi, ::clovMult (clover_dslash_def.h:41-78) */
 int qphix_clov_dslash(qphix_dslash *d, void *res, const void *psi, const void *u,
                       const void *invclov, int isign, int cb);
@@ -245,6 +250,19 @@ int qphix_op_apply(qphix_op *op, void *const res[], const void *const in[], int 
 /* the operator on n single-flavour right-hand sides (no reference counterpart) */
 int qphix_op_apply_multi(qphix_op *op, void *const res[], const void *const in[], int n, int isign,
                          int target_cb);
+/* Multigrid-facing pieces (linearOp.h:26-35; wilson.h:67-86; clover.h:174-251).
+ * Full spinors are [cb 0, cb 1] pairs. M_offdiag: res = -1/2 D in on target_cb
+ * (Wilson, clover). M_diag_inv: clover A_cb^-1 in, Wilson in / (4 + m) (cb 0).
+ * M_diag (clover): A_cb in_cb, cb = -1 for both. M_unprec (clover): the full
+ * operator A in - 1/2 D in on both checkerboards. DslashDir (Wilson, clover):
+ * dslashDir of in_(1-cb) into res_cb for both cb. Other operators:
+ * QPHIX_EINVAL ("not yet implemented", as the reference prints). */
+int qphix_op_m_offdiag(qphix_op *op, void *const res[], const void *const in[], int isign,
+                       int target_cb);
+int qphix_op_m_diag_inv(qphix_op *op, void *const res[], const void *const in[], int isign, int cb);
+int qphix_op_m_diag(qphix_op *op, void *const res[2], const void *const in[2], int isign, int cb);
+int qphix_op_m_unprec(qphix_op *op, void *const res[2], const void *const in[2], int isign);
+int qphix_op_dslash_dir(qphix_op *op, void *const res[2], const void *const in[2], int dir);
 
 /* ---- solvers: AbstractSolver::operator() (abs_solver.h:85-97) for InvCG
  *      (invcg.h:25-694), InvBiCGStab (invbicgstab.h:18-619),
@@ -265,6 +283,22 @@ int qphix_bicgstab_create(qphix_op *M, int max_iters, qphix_solver **s);
 int qphix_richardson_create(qphix_op *M_outer, qphix_solver *inner, double delta,
                             int max_iters, int mdagm, qphix_solver **s);
 int qphix_minvcg_create(qphix_op *M, int max_iters, int max_shifts, qphix_solver **s);
+/* InvMR / InvMRSmoother (invmr.h:19-348): minimal residual with
+ * over-relaxation omega; smoother = 1 runs max_iters iterations without a
+ * residual test and reports rsd_sq_final = -1 */
+int qphix_mr_create(qphix_op *M, int max_iters, double omega, int smoother, qphix_solver **s);
+/* UnprecSolverWrapper (unprec_solver_wrapper.h:16-100) and
+ * AbstractUnprecSolver::operator() (abs_solver.h:324-376), single flavour:
+ * source preparation on solve_cb, the even-odd solve, and reconstruction of
+ * the other checkerboard; x and rhs are [cb 0, cb 1] pairs */
+int qphix_unprec_source_prepare(qphix_op *M, void *const out[], const void *const in_cb[],
+                                const void *const in_other_cb[], int isign, int solve_cb);
+int qphix_unprec_solution_reconstruct(qphix_op *M, void *const out_other_cb[],
+                                      const void *const in_other_cb[], const void *const out_cb[],
+                                      int isign, int solve_cb);
+int qphix_unprec_solve(qphix_solver *eo, void *const x[2], const void *const rhs[2],
+                       double rsd_target, int isign, int verbose, int solve_cb, int resid_type,
+                       qphix_solve_stats *st);
 int qphix_solver_destroy(qphix_solver *s);
 int qphix_solve(qphix_solver *s, void *const x[], const void *const rhs[], double rsd_target,
                 int isign, int verbose, int target_cb, int resid_type,

@@ -130,6 +130,22 @@ def hop(L, u, psi, isign, cb, p=None):
     return out
 
 
+def hop_masked(L, u, psi, isign, cb, mask, p=None):
+    """The hop terms selected by mask (bit 2mu backward, 2mu+1 forward)."""
+    out = np.empty(vh(L) * 24)
+    lib().orc_hop_masked(I4(*L), ptr(u), ptr(psi), ptr(out), isign, cb,
+                         C.byref(p if p is not None else hop_params()), C.c_uint(mask))
+    return out
+
+
+def dslash_dir(L, u, psi, cb, dir, p=None):
+    """Dslash::dslashDir: one hop term, isign = +1 (dslash_body.h:1074-1137)."""
+    out = np.empty(vh(L) * 24)
+    lib().orc_dslash_dir(I4(*L), ptr(u), ptr(psi), ptr(out), cb, dir,
+                         C.byref(p if p is not None else hop_params()))
+    return out
+
+
 def achimbdpsi(L, u, psi, chi, alpha, beta, isign, cb, p=None):
     out = np.empty(vh(L) * 24)
     lib().orc_achimbdpsi(I4(*L), ptr(u), ptr(psi), ptr(chi), ptr(out), alpha,

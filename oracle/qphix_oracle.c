@@ -254,8 +254,12 @@ static void load_cpx(const double *p, int n, cpx *c)
 /* hopping term (dslash_body.h:491-512 coefficients; codegen dslash.cc:332   */
 /* twisted-BC phases applied to every hop of that dimension)                 */
 /* ------------------------------------------------------------------------ */
-void orc_hop(const int L[4], const double *u, const double *psi, double *out,
-             int isign, int cb, const orc_hop_params *p)
+/* hop restricted to the terms whose bit is set in mask: bit 2mu = backward
+ * neighbour term U_mu(x-mu)^+ (1 + isign g_mu) psi(x-mu), bit 2mu+1 = forward
+ * term U_mu(x) (1 - isign g_mu) psi(x+mu) (the accumulate[] order of
+ * dslash_body.h:410-642); mask 0xff is the full hop */
+void orc_hop_masked(const int L[4], const double *u, const double *psi, double *out,
+                    int isign, int cb, const orc_hop_params *p, unsigned mask)
 {
   long vh = orc_vh(L);
   int ocb = 1 - cb;
@@ -270,7 +274,7 @@ void orc_hop(const int L[4], const double *u, const double *psi, double *out,
       cpx ph = cmk(1, 0);
       if (p->use_tbc[mu]) ph = cmk(p->tbc[mu][0], p->tbc[mu][1]);
       /* forward */
-      {
+      if (mask >> (2 * mu + 1) & 1u) {
         double cf = cs;
         if (mu == 3 && c[3] == L[3] - 1) cf *= p->t_boundary;
         long nb = neighbour(L, c, mu, +1);
@@ -287,7 +291,7 @@ void orc_hop(const int L[4], const double *u, const double *psi, double *out,
           }
       }
       /* backward */
-      {
+      if (mask >> (2 * mu) & 1u) {
         double cbk = cs;
         if (mu == 3 && c[3] == 0) cbk *= p->t_boundary;
         long nb = neighbour(L, c, mu, -1);
@@ -309,6 +313,22 @@ void orc_hop(const int L[4], const double *u, const double *psi, double *out,
       out[j * 24 + 2 * k + 1] = acc[k].im;
     }
   }
+}
+
+void orc_hop(const int L[4], const double *u, const double *psi, double *out,
+             int isign, int cb, const orc_hop_params *p)
+{
+  orc_hop_masked(L, u, psi, out, isign, cb, p, 0xffu);
+}
+
+/* Dslash::dslashDir (dslash_body.h:350-358, 1074-1137): one term of the hop
+ * with isign = +1. DPsiDir keeps dir_mask[2mu + recv_from] with
+ * recv_from = 1 - dir % 2, so dir = 2mu takes the forward term (psi(x+mu)) and
+ * dir = 2mu+1 the backward one (psi(x-mu)). */
+void orc_dslash_dir(const int L[4], const double *u, const double *psi, double *out,
+                    int cb, int dir, const orc_hop_params *p)
+{
+  orc_hop_masked(L, u, psi, out, 1, cb, p, 1u << (dir ^ 1));
 }
 
 void orc_achimbdpsi(const int L[4], const double *u, const double *psi,

@@ -54,6 +54,10 @@ typedef struct {
  *              + c_b U_mu(x-mu)^+ (1 + isign g_mu) psi(x-mu),  x on target cb */
 void orc_hop(const int L[4], const double *u, const double *psi, double *out,
              int isign, int cb, const orc_hop_params *p);
+void orc_hop_masked(const int L[4], const double *u, const double *psi, double *out,
+                    int isign, int cb, const orc_hop_params *p, unsigned mask);
+void orc_dslash_dir(const int L[4], const double *u, const double *psi, double *out,
+                    int cb, int dir, const orc_hop_params *p);
 /* out = alpha*chi - beta*hop(psi) */
 void orc_achimbdpsi(const int L[4], const double *u, const double *psi,
                     const double *chi, double *out, double alpha, double beta,

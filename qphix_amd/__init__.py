@@ -329,6 +329,9 @@ class Dslash:
         call("qphix_tmclov_two_flav_achimbdpsi", self.h, _pa(res), _pa(psi), _pa(chi), u.p,
              clov.p, C.c_double(beta), C.c_double(eps), isign, cb)
 
+    def dslash_dir(self, res, psi, u, cb, dir):
+        call("qphix_wilson_dslash_dir", self.h, res.p, psi.p, u.p, cb, dir)
+
     def dslash_multi(self, res, psi, u, isign, cb):
         call("qphix_wilson_dslash_multi", self.h, _pa(res), _pa(psi), len(psi), u.p, isign, cb)
 
@@ -388,6 +391,23 @@ class Operator:
     def apply_multi(self, res, inp, isign, target_cb=1):
         call("qphix_op_apply_multi", self.h, _pa(res), _pa(inp), len(inp), isign, target_cb)
 
+    # multigrid-facing pieces (linearOp.h:26-35, clover.h:174-251); full
+    # spinors are [cb0, cb1] pairs
+    def m_offdiag(self, res, inp, isign, target_cb):
+        call("qphix_op_m_offdiag", self.h, _pa([res]), _pa([inp]), isign, target_cb)
+
+    def m_diag_inv(self, res, inp, isign, cb=0):
+        call("qphix_op_m_diag_inv", self.h, _pa([res]), _pa([inp]), isign, cb)
+
+    def m_diag(self, res, inp, isign, cb=-1):
+        call("qphix_op_m_diag", self.h, _pa(res), _pa(inp), isign, cb)
+
+    def m_unprec(self, res, inp, isign):
+        call("qphix_op_m_unprec", self.h, _pa(res), _pa(inp), isign)
+
+    def dslash_dir(self, res, inp, dir):
+        call("qphix_op_dslash_dir", self.h, _pa(res), _pa(inp), dir)
+
 
 class Solver:
     def __init__(self, h, M, keep=()):
@@ -421,6 +441,27 @@ def InvCG(M, max_iters):
     h = VP()
     call("qphix_cg_create", M.h, max_iters, C.byref(h))
     return Solver(h, M)
+
+
+def InvMR(M, max_iters, omega=1.0):
+    h = VP()
+    call("qphix_mr_create", M.h, max_iters, C.c_double(omega), 0, C.byref(h))
+    return Solver(h, M)
+
+
+def InvMRSmoother(M, max_iters, omega=1.0):
+    h = VP()
+    call("qphix_mr_create", M.h, max_iters, C.c_double(omega), 1, C.byref(h))
+    return Solver(h, M)
+
+
+def unprec_solve(eo_solver, x, rhs, rsd_target, isign=1, solve_cb=1, resid=RESID_RELATIVE,
+                 verbose=False):
+    """x, rhs: [cb0, cb1] field pairs (UnprecSolverWrapper + AbstractUnprecSolver)."""
+    st = SolveStats()
+    call("qphix_unprec_solve", eo_solver.h, _pa(x), _pa(rhs), C.c_double(rsd_target), isign,
+         int(verbose), solve_cb, resid, C.byref(st))
+    return st
 
 
 def InvBlockCG(M, max_iters):

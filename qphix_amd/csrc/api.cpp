@@ -55,8 +55,9 @@ struct qphix_op {
   void *tmp[2];
   void *tmp2[2];
   std::vector<void *> mtmp;  // per-right-hand-side temporaries of qphix_op_apply_multi
+  std::vector<void *> utmp;  // temporaries of the even-odd source preparation / reconstruction
 };
-enum { S_CG = 0, S_BICGSTAB = 1, S_RICHARDSON = 2, S_MINVCG = 3, S_BLOCKCG = 4 };
+enum { S_CG = 0, S_BICGSTAB = 1, S_RICHARDSON = 2, S_MINVCG = 3, S_BLOCKCG = 4, S_MR = 5 };
 struct qphix_solver {
   int kind;
   qphix_op *M;
@@ -68,6 +69,10 @@ struct qphix_solver {
   int mdagm;
   // MInvCG
   int max_shifts;
+  // MR: over-relaxation and whether to stop on the residual (InvMR) or run
+  // max_iters iterations (InvMRSmoother)
+  double omega = 1.0;
+  int mr_terminate = 1;
   // multi-right-hand-side CG work spinors (grown on demand)
   std::vector<void *> mwork;
   // set by a caller that passes x = 0 (the defect-correction loop's inner
@@ -803,7 +808,7 @@ static HopArgs make_hop_args(const qphix_dslash *d, int cb, const void *u, const
 // one kernel-family member applied to n right-hand sides (n = 1: the plain call)
 static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void *const *res,
                         const void *const *psi, const void *const *chi, int n, const void *u, const void *clov,
-                        double coef, double alpha, double mu, double muinv, double rho) {
+                        double coef, double alpha, double mu, double muinv, double rho, int dir = 0) {
   ENSURE_INIT();
   CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(u);
   if (isign != 1 && isign != -1) return fail(QPHIX_EDOMAIN, "isign must be +1 or -1, got %d", isign);
@@ -818,6 +823,7 @@ static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void
   }
   const GeomDesc &g = d->g->d;
   HopArgs a = make_hop_args(d, cb, u, clov, coef, alpha, mu, muinv, rho);
+  a.dir = dir;
   char key[128];
   snprintf(key, sizeof(key), "p%d c%d k%d s%d %dx%dx%dx%d", g.prec, g.c12, kind, isign, g.L[0], g.L[1], g.L[2],
            g.L[3]);
@@ -872,11 +878,11 @@ static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void
 
 static int launch_hop(const qphix_dslash *d, int kind, int isign, int cb, void *res, const void *psi, const void *u,
                       const void *clov, const void *chi, double coef, double alpha, double mu, double muinv,
-                      double rho) {
+                      double rho, int dir = 0) {
   void *r[1] = {res};
   const void *p[1] = {psi};
   const void *c[1] = {chi};
-  return launch_hop_n(d, kind, isign, cb, r, p, c, 1, u, clov, coef, alpha, mu, muinv, rho);
+  return launch_hop_n(d, kind, isign, cb, r, p, c, 1, u, clov, coef, alpha, mu, muinv, rho, dir);
 }
 
 // mode 0: no tuning (natural order, default block); 1: autotune on first use;
@@ -919,6 +925,11 @@ extern "C" const char *qphix_tune_summary(void) {
 
 extern "C" int qphix_wilson_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign, int cb) {
   return launch_hop(d, KD_WILSON, isign, cb, res, psi, u, nullptr, nullptr, 1.0, 0, 0, 0, 0);
+}
+// dslash_def.h:54 / dslash_body.h:350-358 (isign = +1, as the reference)
+extern "C" int qphix_wilson_dslash_dir(qphix_dslash *d, void *res, const void *psi, const void *u, int cb, int dir) {
+  if (dir < 0 || dir > 7) return fail(QPHIX_EDOMAIN, "direction %d (0..7)", dir);
+  return launch_hop(d, KD_DIR, 1, cb, res, psi, u, nullptr, nullptr, 1.0, 0, 0, 0, 0, dir);
 }
 extern "C" int qphix_wilson_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi, const void *u,
                                        double alpha, double beta, int isign, int cb) {
@@ -1056,6 +1067,7 @@ extern "C" int qphix_op_destroy(qphix_op *op) {
     qphix_free(op->tmp2[f]);
   }
   for (void *t : op->mtmp) qphix_free(t);
+  for (void *t : op->utmp) qphix_free(t);
   delete op;
   return QPHIX_OK;
 }
@@ -1172,6 +1184,75 @@ extern "C" int qphix_norm2_multi(const qphix_geom *g, const void *const x[], int
 // ---------------------------------------------------------------------------
 // solvers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// multigrid-facing operator pieces (linearOp.h:26-35; wilson.h:67-86;
+// clover.h:174-251). Fields of a full (unpreconditioned) spinor are passed as
+// two checkerboard pointers [cb 0, cb 1].
+// ---------------------------------------------------------------------------
+static bool op_is_wc(const qphix_op *op) { return op->p.kind == QPHIX_OP_WILSON || op->p.kind == QPHIX_OP_CLOVER; }
+
+// res = -1/2 D in on target_cb (the Wilson hop for both Wilson and clover)
+extern "C" int qphix_op_m_offdiag(qphix_op *op, void *const res[], const void *const in[], int isign, int target_cb) {
+  ENSURE_INIT();
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  TRY(check_cb(target_cb));
+  if (!op_is_wc(op)) return fail(QPHIX_EINVAL, "M_offdiag not yet implemented for this operator");
+  TRY(qphix_wilson_dslash(&op->d, op->tmp[0], in[0], op->p.u[target_cb], isign, target_cb));
+  return qphix_axy(op->g, -0.5, op->tmp[0], res[0]);
+}
+
+// res = A_cb^-1 in (clover; invclov[cb] required) or in / (4 + m) (Wilson, cb 0
+// as the reference asserts)
+extern "C" int qphix_op_m_diag_inv(qphix_op *op, void *const res[], const void *const in[], int isign, int cb) {
+  ENSURE_INIT();
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  TRY(check_cb(cb));
+  (void)isign;
+  if (op->p.kind == QPHIX_OP_WILSON) {
+    if (cb != 0) return fail(QPHIX_EDOMAIN, "Wilson M_diag_inv is defined for cb = 0");
+    return qphix_axy(op->g, 1.0 / (4.0 + op->p.mass), in[0], res[0]);
+  }
+  if (op->p.kind != QPHIX_OP_CLOVER) return fail(QPHIX_EINVAL, "M_diag_inv not yet implemented for this operator");
+  if (!op->p.invclov[cb]) return fail(QPHIX_EINVAL, "inverse clover term of cb %d not set", cb);
+  return qphix_clov_mult(&op->d, res[0], in[0], op->p.invclov[cb]);
+}
+
+// clover: res_cb = A_cb in_cb on cb (cb = -1: both checkerboards)
+extern "C" int qphix_op_m_diag(qphix_op *op, void *const res[2], const void *const in[2], int isign, int cb) {
+  ENSURE_INIT();
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  (void)isign;
+  if (op->p.kind != QPHIX_OP_CLOVER) return fail(QPHIX_EINVAL, "M_diag is defined for the clover operator");
+  if (cb != -1) TRY(check_cb(cb));
+  for (int c = 0; c < 2; c++) {
+    if (cb != -1 && c != cb) continue;
+    if (!op->p.clov[c]) return fail(QPHIX_EINVAL, "clover term of cb %d not set", c);
+    TRY(qphix_clov_mult(&op->d, res[c], in[c], op->p.clov[c]));
+  }
+  return QPHIX_OK;
+}
+
+// clover: res_cb = A_cb in_cb - 1/2 D in_(1-cb) (+ i rho g5 in_cb), both cb
+extern "C" int qphix_op_m_unprec(qphix_op *op, void *const res[2], const void *const in[2], int isign) {
+  ENSURE_INIT();
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  if (op->p.kind != QPHIX_OP_CLOVER) return fail(QPHIX_EINVAL, "M_unprec is defined for the clover operator");
+  for (int c = 0; c < 2; c++) {
+    if (!op->p.clov[c]) return fail(QPHIX_EINVAL, "clover term of cb %d not set", c);
+    TRY(qphix_clov_achimbdpsi(&op->d, res[c], in[1 - c], in[c], op->p.u[c], op->p.clov[c], 0.5, isign, c));
+  }
+  return QPHIX_OK;
+}
+
+// res_cb = term `dir` of the hop (isign +1) of in_(1-cb), both cb
+extern "C" int qphix_op_dslash_dir(qphix_op *op, void *const res[2], const void *const in[2], int dir) {
+  ENSURE_INIT();
+  CHKPTR(op); CHKPTR(res); CHKPTR(in);
+  if (!op_is_wc(op)) return fail(QPHIX_EINVAL, "DslashDir is defined for the Wilson and clover operators");
+  for (int c = 0; c < 2; c++) TRY(qphix_wilson_dslash_dir(&op->d, res[c], in[1 - c], op->p.u[c], c, dir));
+  return QPHIX_OK;
+}
+
 typedef std::vector<void *> SV;  // one spinor per flavour
 
 static int alloc_set(const qphix_geom *g, int nf, SV &v, qphix_solver *s) {
@@ -1478,6 +1559,74 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
   return QPHIX_OK;
 }
 
+// invmr.h:71-221 (InvMRBase::operator(); InvMR stops on the residual,
+// InvMRSmoother runs max_iters iterations and reports rsd_sq_final = -1)
+static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
+                    int verbose, int cb, int rtype, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  const int nf = M->nflav;
+  SV mr, r;
+  if (s->work.empty()) {
+    TRY(alloc_set(g, nf, mr, s));
+    TRY(alloc_set(g, nf, r, s));
+  } else {
+    mr.assign(s->work.begin(), s->work.begin() + nf);
+    r.assign(s->work.begin() + nf, s->work.begin() + 2 * nf);
+  }
+  st->site_flops = 0;
+  st->mv_apps = 0;
+  st->status = QPHIX_OK;
+  if (s->max_iters < 0) return fail(QPHIX_EDOMAIN, "MR: invalid MaxIters = %d", s->max_iters);
+  if (s->max_iters == 0) {
+    st->n_iters = 0;
+    st->rsd_sq_final = -1;
+    return QPHIX_OK;
+  }
+  TRY(op_apply_v(M, r, x, isign, cb));  // r = rhs - M x
+  st->mv_apps++;
+  for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r[f]));
+  const bool term = s->mr_terminate != 0;
+  double norm_chi = 0, rsd_sq = 0, cp = 0;
+  if (term) {
+    TRY(mf_norm2(g, nf, rhs, norm_chi));
+    rsd_sq = rsd_target * rsd_target * (rtype == QPHIX_RESID_RELATIVE ? norm_chi : 1.0);
+    TRY(mf_norm2(g, nf, cv(r), cp));
+    if (verbose) printf("QPhiX MR: iter=0: || r ||^2 = %16.8e Target || r ||^2 = %16.8e\n", cp, rsd_sq);
+    if (cp < rsd_sq) {
+      st->n_iters = 0;
+      st->rsd_sq_final = rtype == QPHIX_RESID_RELATIVE ? cp / norm_chi : cp;
+      return QPHIX_OK;
+    }
+  }
+  int k = 0;
+  bool more = true;
+  while (more) {
+    ++k;
+    TRY(op_apply_v(M, mr, cv(r), isign, cb));
+    st->mv_apps++;
+    double cd[3];  // <Mr, r>, |Mr|^2
+    TRY(mf_cdotnorm(g, nf, cv(mr), cv(r), cd));
+    const double a[2] = {s->omega * cd[0] / cd[2], s->omega * cd[1] / cd[2]};
+    for (int f = 0; f < nf; f++) TRY(qphix_mr_rxupdate(g, x[f], r[f], mr[f], a));
+    if (term) {
+      TRY(mf_norm2(g, nf, cv(r), cp));
+      if (verbose) printf("QPhiX MR: iter=%d: || r ||^2 = %16.8e Target || r ||^2 = %16.8e\n", k, cp, rsd_sq);
+      more = k < s->max_iters && cp > rsd_sq;
+    } else {
+      more = k < s->max_iters;
+    }
+  }
+  st->n_iters = k;
+  if (term) {
+    st->rsd_sq_final = rtype == QPHIX_RESID_RELATIVE ? cp / norm_chi : cp;
+    if (cp > rsd_sq) st->status = QPHIX_ENOCONV;
+  } else {
+    st->rsd_sq_final = -1;
+  }
+  return QPHIX_OK;
+}
+
 // inv_richardson_multiprec.h:42-239
 static int solve_richardson(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                             int verbose, int cb, int rtype, qphix_solve_stats *st) {
@@ -1588,6 +1737,7 @@ extern "C" int qphix_solve(qphix_solver *s, void *const x[], const void *const r
     case S_CG: return solve_cg(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
     case S_BICGSTAB: return solve_bicgstab(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
     case S_RICHARDSON: return solve_richardson(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
+    case S_MR: return solve_mr(s, x, rhs, rsd_target, isign, verbose, target_cb, resid_type, st);
   }
   return fail(QPHIX_EINVAL, "qphix_solve on a multishift solver; use qphix_minvcg_solve");
 }
@@ -2006,6 +2156,77 @@ static int solve_block_cg(qphix_solver *s, void *const *x, const void *const *rh
     st[r].status = broke ? QPHIX_EBREAKDOWN : (conv ? QPHIX_OK : QPHIX_ENOCONV);
   }
   return QPHIX_OK;
+}
+
+// invmr.h:258-348: smoother = 0 is InvMR, 1 is InvMRSmoother
+extern "C" int qphix_mr_create(qphix_op *M, int max_iters, double omega, int smoother, qphix_solver **s) {
+  TRY(qphix_cg_create(M, max_iters, s));
+  (*s)->kind = S_MR;
+  (*s)->omega = omega;
+  (*s)->mr_terminate = smoother ? 0 : 1;
+  return QPHIX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// unpreconditioned solves through the even-odd solver
+// (unprec_solver_wrapper.h:16-100, abs_solver.h:271-376), single flavour.
+// The other-checkerboard inverse diagonal is taken on cb = 1 - solve_cb (the
+// reference calls M_diag_inv with its default cb = 0, the same for the
+// default solve_cb = 1).
+// ---------------------------------------------------------------------------
+static int unprec_tmp(qphix_op *M) {
+  while (M->utmp.size() < 2) {
+    void *p = nullptr;
+    TRY(qphix_alloc_spinor(M->g, &p));
+    M->utmp.push_back(p);
+  }
+  return QPHIX_OK;
+}
+
+// out = in_cb - M_offdiag(solve_cb) M_diag_inv(1 - solve_cb) in_other_cb
+extern "C" int qphix_unprec_source_prepare(qphix_op *M, void *const out[], const void *const in_cb[],
+                                           const void *const in_other_cb[], int isign, int solve_cb) {
+  ENSURE_INIT();
+  CHKPTR(M); CHKPTR(out); CHKPTR(in_cb); CHKPTR(in_other_cb);
+  TRY(check_cb(solve_cb));
+  if (M->nflav != 1) return fail(QPHIX_EDOMAIN, "even-odd source preparation is for single-flavour operators");
+  TRY(unprec_tmp(M));
+  void *t1[1] = {M->utmp[0]}, *t2[1] = {M->utmp[1]};
+  TRY(qphix_copy_spinor(M->g, out[0], in_cb[0]));
+  TRY(qphix_op_m_diag_inv(M, t1, in_other_cb, isign, 1 - solve_cb));
+  TRY(qphix_op_m_offdiag(M, t2, (const void *const *)t1, isign, solve_cb));
+  return qphix_axpy(M->g, -1.0, t2[0], out[0]);
+}
+
+// out_other_cb = M_diag_inv(1 - solve_cb) (in_other_cb - M_offdiag(1 - solve_cb) out_cb)
+extern "C" int qphix_unprec_solution_reconstruct(qphix_op *M, void *const out_other_cb[],
+                                                 const void *const in_other_cb[], const void *const out_cb[],
+                                                 int isign, int solve_cb) {
+  ENSURE_INIT();
+  CHKPTR(M); CHKPTR(out_other_cb); CHKPTR(in_other_cb); CHKPTR(out_cb);
+  TRY(check_cb(solve_cb));
+  if (M->nflav != 1) return fail(QPHIX_EDOMAIN, "even-odd solution reconstruction is for single-flavour operators");
+  TRY(unprec_tmp(M));
+  void *t1[1] = {M->utmp[0]}, *t2[1] = {M->utmp[1]};
+  TRY(qphix_copy_spinor(M->g, t2[0], in_other_cb[0]));
+  TRY(qphix_op_m_offdiag(M, t1, out_cb, isign, 1 - solve_cb));
+  TRY(qphix_axpy(M->g, -1.0, t1[0], t2[0]));
+  return qphix_op_m_diag_inv(M, out_other_cb, (const void *const *)t2, isign, 1 - solve_cb);
+}
+
+// x and rhs are full spinors [cb 0, cb 1]; the even-odd solver works on solve_cb
+extern "C" int qphix_unprec_solve(qphix_solver *eo, void *const x[2], const void *const rhs[2], double rsd_target,
+                                  int isign, int verbose, int solve_cb, int resid_type, qphix_solve_stats *st) {
+  ENSURE_INIT();
+  CHKPTR(eo); CHKPTR(x); CHKPTR(rhs); CHKPTR(st);
+  TRY(check_cb(solve_cb));
+  qphix_op *M = eo->M;
+  void *out_other[1] = {x[1 - solve_cb]}, *out_cb[1] = {x[solve_cb]};
+  const void *in_cb[1] = {rhs[solve_cb]}, *in_other[1] = {rhs[1 - solve_cb]};
+  // the prepared source goes into the other-checkerboard half of x
+  TRY(qphix_unprec_source_prepare(M, out_other, in_cb, in_other, isign, solve_cb));
+  TRY(qphix_solve(eo, out_cb, (const void *const *)out_other, rsd_target, isign, verbose, solve_cb, resid_type, st));
+  return qphix_unprec_solution_reconstruct(M, out_other, in_other, (const void *const *)out_cb, isign, solve_cb);
 }
 
 extern "C" int qphix_block_cg_create(qphix_op *M, int max_iters, qphix_solver **s) {

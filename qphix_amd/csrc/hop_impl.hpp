@@ -20,7 +20,9 @@
 
 namespace qphix_amd {
 
-enum HopKind { KD_WILSON = 0, KA_WILSON, KD_CLOVER, KA_CLOVER, KD_TM, KA_TM, KD_FULL, NUM_HOP_KINDS };
+// KD_DIR: one term of the hop (Dslash::dslashDir, dslash_body.h:1074-1137),
+// a.dir = 2 mu takes the forward neighbour term, 2 mu + 1 the backward one
+enum HopKind { KD_WILSON = 0, KA_WILSON, KD_CLOVER, KA_CLOVER, KD_TM, KA_TM, KD_FULL, KD_DIR, NUM_HOP_KINDS };
 
 
 struct HopArgs {
@@ -41,6 +43,7 @@ struct HopArgs {
   // walked z-block by z-block (bz planes), t innermost, so consecutive work-
   // groups of one XCD reuse the source spinor planes they share
   int bz, slabs;
+  int dir;  // KD_DIR only
   // multi-RHS: nrhs > 1 applies the operator to psis[r] -> outs[r] (chis[r]);
   // the right-hand side is the fastest index of the work-group order, so the
   // work-groups sharing a site range (and its links and clover) run together
@@ -347,6 +350,27 @@ __device__ inline void hop_dim_r(const HopArgs &a, const void *const (&psi)[R], 
   }
 }
 
+// a single hop term (direction MU, forward or backward) for R right-hand sides
+template <int P, bool C12, int ISIGN, int MU, bool FWD, int R>
+__device__ inline void hop_term_r(const HopArgs &a, const void *const (&psi)[R], unsigned s, unsigned sn,
+                                  typename PT<P>::calc coef, cx<typename PT<P>::calc> (&acc)[R][4][3]) {
+  typedef typename PT<P>::calc T;
+  typedef GaugeLayout<P, C12> GL;
+  const bool tbc = (a.tbc_mask >> MU) & 1;
+  const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
+  cx<T> U[3][3];
+  if constexpr (GL::per_link) {
+    T lv[GL::link_reals];
+    load_range<P, GL::nw, (2 * MU + (FWD ? 1 : 0)) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv, U);
+  } else {
+    T lv[2 * GL::link_reals];
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv + (FWD ? GL::link_reals : 0), U);
+  }
+  hop_half_r<P, C12, ISIGN, MU, FWD, R>(U, a, psi, sn, coef, tr, ti, acc);
+}
+
 // packed Hermitian clover block B applied to R spinors (block loaded once)
 template <int P, int B, int R, typename T>
 __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s, const cx<T> (&in)[R][4][3],
@@ -493,7 +517,18 @@ __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *cons
 
   cx<T> acc[R][4][3];
   hop_init_r<P, ISIGN, KIND, R>(a, s, chi, acc);
-  if constexpr (R == 1) {
+  if constexpr (KIND == KD_DIR) {
+    switch (a.dir) {
+      case 0: hop_term_r<P, C12, ISIGN, 0, true, R>(a, psi, s, n.xf, cs, acc); break;
+      case 1: hop_term_r<P, C12, ISIGN, 0, false, R>(a, psi, s, n.xb, cs, acc); break;
+      case 2: hop_term_r<P, C12, ISIGN, 1, true, R>(a, psi, s, n.yf, cs, acc); break;
+      case 3: hop_term_r<P, C12, ISIGN, 1, false, R>(a, psi, s, n.yb, cs, acc); break;
+      case 4: hop_term_r<P, C12, ISIGN, 2, true, R>(a, psi, s, n.zf, cs, acc); break;
+      case 5: hop_term_r<P, C12, ISIGN, 2, false, R>(a, psi, s, n.zb, cs, acc); break;
+      case 6: hop_term_r<P, C12, ISIGN, 3, true, R>(a, psi, s, n.tf, ctf, acc); break;
+      default: hop_term_r<P, C12, ISIGN, 3, false, R>(a, psi, s, n.tb, ctb, acc); break;
+    }
+  } else if constexpr (R == 1) {
     hop_dim<P, C12, ISIGN, 0>(a, psi[0], s, n.xb, n.xf, cs, cs, acc[0]);
     hop_dim<P, C12, ISIGN, 1>(a, psi[0], s, n.yb, n.yf, cs, cs, acc[0]);
     hop_dim<P, C12, ISIGN, 2>(a, psi[0], s, n.zb, n.zf, cs, cs, acc[0]);

@@ -17,6 +17,13 @@ static hipError_t launch_hop3(int kind, const HopArgs &a, int block, hipStream_t
     case KD_TM: hop_kernel<P, C12, ISIGN, KD_TM><<<grid, block, 0, st>>>(a); break;
     case KA_TM: hop_kernel<P, C12, ISIGN, KA_TM><<<grid, block, 0, st>>>(a); break;
     case KD_FULL: hop_kernel<P, C12, ISIGN, KD_FULL><<<grid, block, 0, st>>>(a); break;
+    case KD_DIR:  // isign = +1 only (the reference hard-wires it)
+      if constexpr (ISIGN == 1) {
+        if (a.nrhs > 1) return hipErrorInvalidValue;
+        hop_kernel<P, C12, ISIGN, KD_DIR><<<grid, block, 0, st>>>(a);
+        break;
+      }
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

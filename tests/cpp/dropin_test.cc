@@ -157,6 +157,53 @@ int main()
     }
     check("odd_extent_throws_domain_error", thrown ? 0.0 : 1.0, 0.0);
   }
+  // multigrid-facing pieces and the unpreconditioned solve through the
+  // even-odd solver (clover.h:174-251, unprec_solver_wrapper.h, abs_solver.h)
+  {
+    typedef FullSpinor<double, 8, 4, false> Full;
+    Full xf(geom), bf(geom);
+    detail::check(qphix_gen_spinor(geom.handle(), bf.getCBData(0), 21, 0), "gen_spinor");
+    detail::check(qphix_gen_spinor(geom.handle(), bf.getCBData(1), 22, 1), "gen_spinor");
+    zeroSpinor(xf.getCBData(0), geom, 1);
+    zeroSpinor(xf.getCBData(1), geom, 1);
+    InvBiCGStab<double, 8, 4, false> eo(M, 1000);
+    UnprecSolverWrapper<double, 8, 4, false, EvenOddCloverOperator<double, 8, 4, false>> us(eo, M);
+    int it;
+    double rsd;
+    unsigned long fl, mv;
+    us(&xf, &bf, 1e-11, it, rsd, fl, mv, 1, false);
+    // odd rows: A_oo x_o - 1/2 D x_e = b_o
+    ClovDslash<double, 8, 4, false> D(&geom, t_bc, 1.0, 1.0);
+    D.clovMult(r, xf.getCBData(1), clov);
+    M.M_offdiag(t, xf.getCBData(0), 1, 1);
+    ypeqx(t, r, geom, 1);
+    double n, bo;
+    xmyNorm2Spinor(t, bf.getCBData(1), r, n, geom, 1);
+    norm2Spinor(bo, bf.getCBData(1), geom, 1);
+    check("unprec_odd_rows", std::sqrt(n / bo), 1e-9);
+    // even rows: x_e = A_ee^-1 (b_e + 1/2 D x_o)
+    M.M_offdiag(t, xf.getCBData(1), 1, 0);
+    xmyNorm2Spinor(r, bf.getCBData(0), t, n, geom, 1);
+    M.M_diag_inv(x, r, 1, 0);
+    double xe;
+    xmyNorm2Spinor(r, x, xf.getCBData(0), n, geom, 1);
+    norm2Spinor(xe, xf.getCBData(0), geom, 1);
+    check("unprec_even_rows", std::sqrt(n / xe), 1e-13);
+  }
+  // InvMR on the clover operator (invmr.h)
+  {
+    zeroSpinor(x, geom, 1);
+    InvMR<double, 8, 4, false> mr(M, 5000, 1.0);
+    int it;
+    double rsd;
+    unsigned long fl, mv;
+    mr(x, b, 1e-9, it, rsd, fl, mv, 1, false);
+    M(t, x, 1);
+    double n;
+    xmyNorm2Spinor(r, b, t, n, geom, 1);
+    check("mr_true_resid", std::sqrt(n / bnorm), 1.5e-9);
+  }
+
   for (int cb = 0; cb < 2; cb++) {
     geom.free(u[cb]);
     geom32.free(u32[cb]);

@@ -149,6 +149,59 @@ def test_plane_wave(orc, isign, tb):
         assert np.abs(cplx(out).reshape(-1, 4, 3) - expect).max() < 1e-12
 
 
+def test_dslash_dir_plane_wave_and_sum(orc):
+    """dslashDir (dslash_body.h:1074-1137): on the free field a plane wave picks
+    up exactly one term of the hop kernel per direction, dir = 2mu the forward
+    term c e^{ip_mu} (1 - g_mu), dir = 2mu+1 the backward one; on a random
+    gauge field the eight directions add up to the isign = +1 hop."""
+    L = [4, 6, 4, 8]
+    g = gammas(orc)
+    u = unit_gauge(L)
+    p = 2 * math.pi * np.array([1, 2, 3, 1]) / np.array(L, dtype=float)
+    p[3] += math.pi / L[3]  # antiperiodic in t
+    rng = np.random.default_rng(4)
+    eta = rng.normal(size=(4, 3)) + 1j * rng.normal(size=(4, 3))
+    hp = orc.hop_params(-1.0, 1.0, 0.7)
+    for cb in (0, 1):
+        src = coords_of_cb(L, 1 - cb)
+        psi = (np.exp(1j * src @ p)[:, None, None] * eta[None]).view(np.float64).ravel().copy()
+        tgt = coords_of_cb(L, cb)
+        for d in range(8):
+            mu, fwd = d // 2, d % 2 == 0
+            c = 0.7 if mu == 3 else 1.0
+            K = c * (np.exp(1j * p[mu]) * (np.eye(4) - g[mu]) if fwd
+                     else np.exp(-1j * p[mu]) * (np.eye(4) + g[mu]))
+            out = orc.dslash_dir(L, u, psi, cb, d, hp)
+            expect = np.exp(1j * tgt @ p)[:, None, None] * (K @ eta)[None]
+            assert np.abs(cplx(out).reshape(-1, 4, 3) - expect).max() < 1e-12, d
+    ur = orc.gen_gauge(L, 9, 0.4)
+    psi = orc.gen_spinor(L, 10, 0)
+    tot = sum(orc.dslash_dir(L, ur, psi, 1, d, hp) for d in range(8))
+    assert np.abs(tot - orc.hop(L, ur, psi, 1, 1, hp)).max() < 1e-13
+
+
+def test_dslash_dir_point_source(orc):
+    """dir = 2mu moves a point source at y to the site y - mu (the result at x
+    reads psi(x + mu)); dir = 2mu + 1 to y + mu."""
+    L = [4, 4, 4, 8]
+    vh = 4 * 4 * 4 * 8 // 2
+    u = orc.gen_gauge(L, 3, 0.4)
+    ycb = 0
+    j = 37
+    yc = coords_of_cb(L, ycb)[j]
+    psi = np.zeros(vh * 24)
+    psi[j * 24 + 5] = 1.0
+    tgt = coords_of_cb(L, 1 - ycb)
+    for d in range(8):
+        mu, fwd = d // 2, d % 2 == 0
+        out = orc.dslash_dir(L, u, psi, 1 - ycb, d)
+        nz = np.nonzero(np.abs(out.reshape(vh, 24)).sum(axis=1))[0]
+        assert len(nz) == 1, d
+        x = tgt[nz[0]].copy()
+        x[mu] = (x[mu] + (1 if fwd else -1)) % L[mu]
+        assert np.array_equal(x, yc), d
+
+
 def random_su3(rng, n):
     G = rng.normal(size=(n, 3, 3)) + 1j * rng.normal(size=(n, 3, 3))
     Q, R = np.linalg.qr(G)
