@@ -29,6 +29,7 @@ CSW = 1.0
 HBM_PEAK_GBS = 8000.0
 BYTES_SITE = {("f32", True): 864, ("f32", False): 1056, ("f64", False): 2112, ("f16", True): 448}
 FLOPS_SITE_CLOVER = 1824
+RHS_TILE = []  # forced multi-RHS tiling (--rhs-tile), empty: autotuned
 
 
 def log(msg):
@@ -86,6 +87,9 @@ def main():
     ap.add_argument("--no-block", action="store_true")
     ap.add_argument("--no-multi", action="store_true",
                     help="skip the 12-RHS launches (they share the headline kernel's symbol)")
+    ap.add_argument("--rhs-tile", default=None,
+                    help="reg,rt: fixed multi-RHS tiling (register tile, sources per launch) instead of "
+                         "autotuning (profiler runs)")
     ap.add_argument("--hop-config", default=None,
                     help="block,bz,slabs: fixed hop-kernel launch configuration instead of "
                          "autotuning (used for profiler runs so every launch is identical)")
@@ -100,6 +104,10 @@ def main():
     if args.hop_config:
         blk, bz, sl = (int(v) for v in args.hop_config.split(","))
         q.call("qphix_tune_config", 2, blk, bz, sl)
+    if args.rhs_tile:
+        reg, rt = (int(v) for v in args.rhs_tile.split(","))
+        q.call("qphix_tune_rhs", reg, rt)
+        RHS_TILE[:] = [reg, rt]
     vh = LATT[0] * LATT[1] * LATT[2] * LATT[3] // 2
     t_setup = time.perf_counter()
     g64 = q.Geometry(LATT, q.F64, False)
@@ -235,7 +243,7 @@ def multi_rhs(q, D, g32, u32, inv32, nr, reps, per_ms, timer):
     for _ in range(reps):
         D.clov_dslash_multi(outs, psis, u32[0], inv32, 1, 0)
     ms_m = timer.stop() / reps
-    q.call("qphix_tune_rhs", 0, 0)
+    q.call("qphix_tune_rhs", *(RHS_TILE or (0, 0)))
     issued = 3072.0 * vh * nm   # MFMA flops issued per launch (incl. the block-diagonal zeros)
     useful = 1152.0 * vh * nm   # 8 hops x 2 spin components x 6 x 6 real multiply-adds
     gbs_m = (8 * 12 + 72 + nm * 48) * 4 * vh / (ms_m * 1e-3) / 1e9
