@@ -1284,6 +1284,16 @@ static int mf_cdot(const qphix_geom *g, int nf, const void *const *l, const void
   }
   return QPHIX_OK;
 }
+// true when every flavour of x is exactly zero (one norm sweep): the solvers
+// then take r = rhs instead of applying the operator to the zero vector, the
+// common start of a solve from x = 0
+static int mf_is_zero(const qphix_geom *g, int nf, const void *const *x, bool &zero) {
+  double n;
+  TRY(mf_norm2(g, nf, x, n));
+  zero = n == 0.0;
+  return QPHIX_OK;
+}
+
 // <l, r> and |l|^2 in one sweep, summed over flavours
 static int mf_cdotnorm(const qphix_geom *g, int nf, const void *const *l, const void *const *r, double o[3]) {
   o[0] = o[1] = o[2] = 0;
@@ -1395,7 +1405,9 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
   double rsd_sq = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= chi_sq;
   double cp = 0;
-  if (s->x_zero) {
+  bool xz = s->x_zero != 0;
+  if (!xz) TRY(mf_is_zero(g, nf, (const void *const *)x, xz));
+  if (xz) {
     for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r[f], rhs[f]));
     cp = chi_sq;
   } else {
@@ -1478,7 +1490,9 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
   double rsd_sq = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= rhs_sq;
   double r_norm;
-  if (s->x_zero) {
+  bool xz = s->x_zero != 0;
+  if (!xz) TRY(mf_is_zero(g, nf, (const void *const *)x, xz));
+  if (xz) {
     for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r0[f], rhs[f]));
     r_norm = rhs_sq;
   } else {
@@ -1583,9 +1597,15 @@ static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, dou
     st->rsd_sq_final = -1;
     return QPHIX_OK;
   }
-  TRY(op_apply_v(M, r, x, isign, cb));  // r = rhs - M x
-  st->mv_apps++;
-  for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r[f]));
+  bool xz;
+  TRY(mf_is_zero(g, nf, (const void *const *)x, xz));
+  if (xz) {  // r = rhs - M x
+    for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r[f], rhs[f]));
+  } else {
+    TRY(op_apply_v(M, r, x, isign, cb));
+    st->mv_apps++;
+    for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r[f]));
+  }
   const bool term = s->mr_terminate != 0;
   double norm_chi = 0, rsd_sq = 0, cp = 0;
   if (term) {
@@ -1673,7 +1693,14 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
     }
     return QPHIX_OK;
   };
-  TRY(true_resid());
+  bool xz;
+  TRY(mf_is_zero(g, nf, (const void *const *)x, xz));
+  if (xz) {  // r = rhs - M 0
+    for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r[f], rhs[f]));
+    r_norm_sq = rhs_sq;
+  } else {
+    TRY(true_resid());
+  }
   if (verbose) printf("RICHARDSON: Initial || r ||^2 = %16.8e Target || r ||^2 = %16.8e\n", r_norm_sq, rsd_t);
   if (r_norm_sq < rsd_t) {
     st->n_iters = 0;
