@@ -227,7 +227,8 @@ def multi_rhs(q, D, g32, u32, inv32, nr, reps, per_ms, timer):
     gbs = bytes_multi / (ms_multi * 1e-3) / 1e9
     log(f"clover dslash f32 c12, {nr} rhs: {ms_multi:.3f} ms ({ms_multi / nr:.4f} ms per rhs), "
         f"{gbs:.1f} GB/s algorithmic")
-    out = {"ms": round(ms_multi, 4), "ms_per_rhs": round(ms_multi / nr, 5),
+    out = {"tuned": [ln for ln in q.tune_summary().splitlines() if f" n{nr}" in ln],
+           "ms": round(ms_multi, 4), "ms_per_rhs": round(ms_multi / nr, 5),
            "gbs_algorithmic": round(gbs, 1),
            "speedup_per_rhs_vs_single": round(per_ms / (ms_multi / nr), 3)}
     # the matrix-core kernel on 16 right-hand sides: each link multiplies the
