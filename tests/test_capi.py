@@ -56,3 +56,15 @@ def test_dropin_program_links():
         pytest.skip("drop-in test program not built")
     out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
     assert "libqphix_amd.so" in out and "not found" not in out.split("libqphix_amd.so")[1].split("\n")[0]
+
+
+def test_tune_rhs_validation():
+    """qphix_tune_rhs rejects register tiles other than 1, 2, 4, 8 and launch
+    sizes that are not multiples of the tile (host-side checks, no GPU)."""
+    lib = q.lib()
+    assert lib.qphix_tune_rhs(3, 6) == q.EDOMAIN
+    assert lib.qphix_tune_rhs(4, 6) == q.EDOMAIN
+    assert lib.qphix_tune_rhs(8, 40) == q.EDOMAIN
+    assert lib.qphix_tune_rhs(2, 1) == q.EDOMAIN
+    assert lib.qphix_tune_rhs(0, 0) == q.OK
+    assert lib.qphix_tune_config(2, 100, 0, 1) == q.EDOMAIN  # block not a multiple of 64
