@@ -2424,6 +2424,11 @@ struct qphix_batch {
   std::vector<void *> tmp, mp, mmp, p, r;  // per-system work spinors
   std::vector<int> off;                    // chunk / work-group offsets (256 sites)
   int *d_off = nullptr, *d_active = nullptr;
+  // hop segments: one per system, or (tile > 1) one per group of `tile`
+  // consecutive systems that share one operator, applied as a register tile
+  int tile = 1, nhseg = 0;
+  std::vector<int> hoff;
+  int *d_hoff = nullptr;
   HopSeg *d_hop = nullptr;                 // 4 stages x n
   SegVec *d_seg = nullptr;                 // 8 step kinds x n
   CgState *d_state = nullptr;
@@ -2433,7 +2438,7 @@ struct qphix_batch {
 
 static int batch_free_dev(qphix_batch *b) {
   void *ptrs[] = {b->d_off, b->d_active, b->d_hop, b->d_seg, b->d_state, b->d_alpha, b->d_beta,
-                  b->d_d, b->d_cp, b->d_rsd, b->d_partials};
+                  b->d_d, b->d_cp, b->d_rsd, b->d_partials, b->d_hoff};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   return QPHIX_OK;
@@ -2474,6 +2479,25 @@ extern "C" int qphix_batch_create(qphix_op *const ops[], int nsys, qphix_batch *
   }
   b->off.assign(nsys + 1, 0);
   for (int i = 0; i < nsys; i++) b->off[i + 1] = b->off[i] + (int)((ops[i]->g->d.vh + 255) / 256);
+  // Systems that come in groups of kBatchTile sharing one operator (several
+  // right-hand sides per lattice) can run their hops as a register tile, which
+  // reads links and clover once per group. That pays where the hops are
+  // bandwidth-bound; on small lattices the batch is latency-bound and one
+  // source per work-item (4x the work-items) is faster. Default: tile when
+  // every lattice has at least 2^17 sites per parity; QPHIX_BATCH_TILE=0 / 1
+  // forces it off / on (where the grouping allows).
+  const char *te = getenv("QPHIX_BATCH_TILE");
+  bool want = true;
+  for (int i = 0; i < nsys; i++) want = want && ops[i]->g->d.vh >= (1L << 17);
+  if (te && te[0] == '0') want = false;
+  if (te && te[0] == '1') want = true;
+  b->tile = want && nsys % kBatchTile == 0 ? kBatchTile : 1;
+  for (int i = 0; i < nsys && b->tile > 1; i++)
+    if (ops[i] != ops[i - i % kBatchTile]) b->tile = 1;
+  b->nhseg = nsys / b->tile;
+  b->hoff.assign(b->nhseg + 1, 0);
+  for (int h = 0; h < b->nhseg; h++)
+    b->hoff[h + 1] = b->hoff[h] + (int)((ops[h * b->tile]->g->d.vh + 255) / 256);
   std::vector<void *> *sets[5] = {&b->tmp, &b->mp, &b->mmp, &b->p, &b->r};
   for (auto *v : sets)
     for (int i = 0; i < nsys; i++) {
@@ -2493,13 +2517,15 @@ extern "C" int qphix_batch_create(qphix_op *const ops[], int nsys, qphix_batch *
             hipMalloc(&b->d_d, sizeof(double) * nn) == hipSuccess &&
             hipMalloc(&b->d_cp, sizeof(double) * nn) == hipSuccess &&
             hipMalloc(&b->d_rsd, sizeof(double) * nn) == hipSuccess &&
-            hipMalloc(&b->d_partials, sizeof(double) * (size_t)b->off[nsys]) == hipSuccess;
+            hipMalloc(&b->d_partials, sizeof(double) * (size_t)b->off[nsys]) == hipSuccess &&
+            hipMalloc(&b->d_hoff, sizeof(int) * (size_t)(b->nhseg + 1)) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     qphix_batch_destroy(b);
     return fail(QPHIX_ENOMEM, "batch device allocation failed");
   }
-  if (hipMemcpy(b->d_off, b->off.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(b->d_off, b->off.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_hoff, b->hoff.data(), sizeof(int) * (size_t)(b->nhseg + 1), hipMemcpyHostToDevice) != hipSuccess) {
     qphix_batch_destroy(b);
     return fail(QPHIX_EHIP, "batch offsets upload failed");
   }
@@ -2560,6 +2586,14 @@ static int batch_grid(const qphix_batch *b) {
 
 static hipError_t batch_hop(const qphix_batch *b, int stage, int kind, int isign) {
   const HopSeg *segs = b->d_hop + (size_t)stage * b->n;
+  if (b->tile > 1) {
+    const int total = b->hoff[b->nhseg], grid = total < 2048 ? total : 2048;
+    switch (b->prec) {
+      case P64: return launch_hop_batch_tile_prec<P64>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
+      case P32: return launch_hop_batch_tile_prec<P32>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
+      default: return launch_hop_batch_tile_prec<P16>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
+    }
+  }
   switch (b->prec) {
     case P64: return launch_hop_batch_prec<P64>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
     case P32: return launch_hop_batch_prec<P32>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
@@ -2601,14 +2635,27 @@ extern "C" int qphix_batch_cg(qphix_batch *b, void *const x[], const void *const
     const GeomDesc &g = b->ops[i]->g->d;
     HopArgs a[2];
     int k2[2];
+    // hop segment of system i: its own (tile 1), or slot j of its group's
+    // register tile (the group's first system carries the operator)
+    const int h = i / b->tile, j = i % b->tile;
+    auto put = [&](int stage, const HopArgs &ai) {
+      HopSeg &sg = hop[(size_t)stage * n + (b->tile > 1 ? h : i)];
+      if (j == 0) sg = HopSeg{ai, i, b->tile};
+      if (b->tile > 1) {
+        sg.a.nrhs = b->tile;
+        sg.a.psis[j] = ai.psi;
+        sg.a.outs[j] = ai.out;
+        sg.a.chis[j] = ai.chi;
+      }
+    };
     TRY(op_stages(b->ops[i], isign, t, b->p[i], b->tmp[i], b->mp[i], k2, a));   // mp = M p
-    hop[0 * n + i] = HopSeg{a[0], i};
-    hop[1 * n + i] = HopSeg{a[1], i};
+    put(0, a[0]);
+    put(1, a[1]);
     kinds[0] = k2[0];
     kinds[1] = k2[1];
     TRY(op_stages(b->ops[i], -isign, t, b->mp[i], b->tmp[i], b->mmp[i], k2, a));  // mmp = M^+ mp
-    hop[2 * n + i] = HopSeg{a[0], i};
-    hop[3 * n + i] = HopSeg{a[1], i};
+    put(2, a[0]);
+    put(3, a[1]);
     kinds[2] = k2[0];
     kinds[3] = k2[1];
     auto sv = [&](const void *xx, const void *yy, void *o0, void *o1) { return SegVec{xx, yy, o0, o1, g.vh, g.vs, i}; };
@@ -2621,6 +2668,7 @@ extern "C" int qphix_batch_cg(qphix_batch *b, void *const x[], const void *const
     seg[6 * n + i] = sv(x[i], nullptr, nullptr, nullptr);            // |x|^2
   }
   HIPCHK(hipMemcpyAsync(b->d_hop, hop.data(), sizeof(HopSeg) * hop.size(), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));  // the host tables are local
   HIPCHK(hipMemcpyAsync(b->d_seg, seg.data(), sizeof(SegVec) * seg.size(), hipMemcpyHostToDevice, g_stream));
   // set-up: r = rhs - M^+M x (the operator only where x != 0), p = r, targets
   std::vector<double> rn(n), xn(n), rsd(n);

@@ -6,4 +6,6 @@ template hipError_t launch_hop_prec<P16>(int, bool, int, const HopArgs &, int, h
 template hipError_t launch_hop_tile_prec<P16>(int, bool, int, int, const HopArgs &, int, hipStream_t);
 template hipError_t launch_hop_batch_prec<P16>(int, bool, int, const HopSeg *, const int *, int, const int *,
                                                   int, hipStream_t);
+template hipError_t launch_hop_batch_tile_prec<P16>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                       int, hipStream_t);
 }

@@ -10,4 +10,6 @@ hipError_t launch_hop_mfma_f32(int kind, bool c12, int isign, const HopArgs &a, 
 }
 template hipError_t launch_hop_batch_prec<P32>(int, bool, int, const HopSeg *, const int *, int, const int *,
                                                   int, hipStream_t);
+template hipError_t launch_hop_batch_tile_prec<P32>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                       int, hipStream_t);
 }

@@ -613,9 +613,11 @@ __global__ void __launch_bounds__(256) hop_tile_kernel(const HopArgs a) {
 // form), so the launch shape does not depend on the batch and can be captured
 // once in a graph. Segments whose system has converged (active[sys] == 0) are
 // skipped.
+constexpr int kBatchTile = 4;  // systems per segment of the tiled batched hop
 struct HopSeg {
   HopArgs a;
-  int sys;
+  int sys;   // first system of the segment
+  int nsys;  // tiled form: systems sys .. sys + nsys - 1 share the operator (a.psis / outs / chis)
 };
 template <int P, bool C12, int ISIGN, int KIND>
 __global__ void __launch_bounds__(256) hop_batch_kernel(const HopSeg *segs, const int *blk_off, int nseg,
@@ -631,10 +633,49 @@ __global__ void __launch_bounds__(256) hop_batch_kernel(const HopSeg *segs, cons
   }
 }
 
+// Batched hop with a register tile: each segment holds R systems that share
+// one operator (the same lattice, links and clover, R right-hand sides), so
+// every work-item applies the links and clover of its site to all R sources
+// (hop_body_r). A segment is skipped once all its systems have converged.
+template <int P, bool C12, int ISIGN, int KIND, int R>
+__global__ void __launch_bounds__(256) hop_batch_tile_kernel(const HopSeg *segs, const int *blk_off, int nseg,
+                                                              const int *active) {
+  const int total = blk_off[nseg];
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    int g = 0;
+    while (g + 1 < nseg && blk_off[g + 1] <= b) g++;
+    const HopSeg &sg = segs[g];
+    if (active) {
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < R; j++) any = any || active[sg.sys + j] != 0;
+      if (!any) continue;
+    }
+    const unsigned k = (unsigned)(b - blk_off[g]) * blockDim.x + threadIdx.x;
+    if (k < (unsigned)sg.a.vh) {
+      const void *psi[R], *chi[R];
+      void *out[R];
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        psi[j] = sg.a.psis[j];
+        out[j] = sg.a.outs[j];
+        chi[j] = sg.a.chis[j];
+      }
+      hop_body_r<P, C12, ISIGN, KIND, R>(sg.a, k, psi, out, chi);
+    }
+  }
+}
+
 template <int P>
 hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int block, hipStream_t st);
 template <int P>
 hipError_t launch_hop_tile_prec(int kind, bool c12, int isign, int rt, const HopArgs &a, int block, hipStream_t st);
+template <int P>
+hipError_t launch_hop_batch_tile_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
+                                      const int *active, int grid, hipStream_t st);
+template <int P>
+hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
+                                 const int *active, int grid, hipStream_t st);
 // matrix-core multi-RHS launch (FP32 storage, a.nrhs a multiple of 8)
 hipError_t launch_hop_mfma_f32(int kind, bool c12, int isign, const HopArgs &a, hipStream_t st);
 

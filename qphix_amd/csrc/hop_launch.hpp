@@ -86,6 +86,33 @@ static hipError_t launch_batch3(int kind, const HopSeg *segs, const int *off, in
   return hipGetLastError();
 }
 
+// tiled batch: every segment carries kBatchTile systems sharing one operator
+template <int P, bool C12, int ISIGN>
+static hipError_t launch_batch_tile3(int kind, const HopSeg *segs, const int *off, int nseg, const int *active,
+                                     int grid, hipStream_t st) {
+  switch (kind) {
+#define BT(K) \
+  case K: hop_batch_tile_kernel<P, C12, ISIGN, K, kBatchTile><<<grid, kBatchBlock, 0, st>>>(segs, off, nseg, active); break;
+    BT(KD_WILSON) BT(KA_WILSON) BT(KD_CLOVER) BT(KA_CLOVER) BT(KD_TM) BT(KA_TM) BT(KD_FULL)
+#undef BT
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int P>
+hipError_t launch_hop_batch_tile_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
+                                      const int *active, int grid, hipStream_t st) {
+  if (c12)
+    return isign > 0 ? launch_batch_tile3<P, true, 1>(kind, segs, off, nseg, active, grid, st)
+                     : launch_batch_tile3<P, true, -1>(kind, segs, off, nseg, active, grid, st);
+  if constexpr (P != P16) {
+    return isign > 0 ? launch_batch_tile3<P, false, 1>(kind, segs, off, nseg, active, grid, st)
+                     : launch_batch_tile3<P, false, -1>(kind, segs, off, nseg, active, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 // segs / off / active are device arrays; off has nseg + 1 entries
 template <int P>
 hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
