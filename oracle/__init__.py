@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
-and by the CPU-baseline leg of ``bench.py``, always as the checker.  Arrays
+and by the CPU-baseline leg of ``bench.py``, always to produce reference
+results, never as the product path.  Arrays
 are float64 numpy arrays in the canonical layout documented in
 ``oracle/qphix_oracle.h``.
 """

@@ -1,7 +1,8 @@
 /*
  * qphix_oracle.c -- FP64 CPU restatement of the QPhiX Dslash / clover /
  * twisted-mass / BLAS / solver algorithms.  TEST INFRASTRUCTURE ONLY: the
- * checker for tests/, the smoke check and bench.py's CPU baseline.  Layout and
+ * reference results of tests/, of the smoke check and of bench.py's CPU
+ * baseline; never part of the product path.  Layout and
  * conventions are described in qphix_oracle.h.
  *
  * The hopping term is written with explicit dense 4x4 gamma matrices (no

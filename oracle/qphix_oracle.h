@@ -3,8 +3,8 @@
  * Dslash path, its fused BLAS and its Krylov solvers.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, single-threaded
- * restatement of the QPhiX algorithms, used as the checker in tests/, in the
- * smoke check and as the CPU baseline leg of bench.py.  The product path
+ * restatement of the QPhiX algorithms that provides the reference results of
+ * tests/ and of the smoke check, and the CPU baseline leg of bench.py.  The product path
  * (libqphix_amd.so) never calls it.
  *
  * Host layout (the "canonical" layout, the same one the C ABI packers accept;

@@ -1,5 +1,5 @@
-"""Pins the CPU oracle (oracle/qphix_oracle.c) before it is trusted as the
-checker for the HIP path.
+"""Pins the CPU oracle (oracle/qphix_oracle.c) before its results are trusted
+as the reference for the HIP path.
 
 Each check is chosen so that a plausible slip in the oracle (a swapped
 projector sign, U instead of U^+, a link taken at x instead of x-mu, a
