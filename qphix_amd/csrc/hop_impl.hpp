@@ -103,11 +103,11 @@ template <int P, bool C12> struct GaugeLayout {
 // The twisted-boundary phase (tr, ti) is applied unconditionally, (1, 0) when
 // the dimension has none: a branch here would split the kernel into basic
 // blocks and stop the scheduler from issuing the next hop's loads early.
+// apply_hop_h takes the projected half spinor h = (1 -/+ isign g_mu) psi.
 template <int MU, int S, bool FWD, bool C12, typename T>
-__device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3], T coef, T tr, T ti,
-                                 cx<T> (&acc)[4][3]) {
-  cx<T> h[2][3], uh[2][3];
-  project<MU, S>(psi, h);
+__device__ inline void apply_hop_h(const cx<T> (&U)[3][3], const cx<T> (&h)[2][3], T coef, T tr, T ti,
+                                   cx<T> (&acc)[4][3]) {
+  cx<T> uh[2][3];
 #pragma unroll
   for (int r = 0; r < 2; r++)
 #pragma unroll
@@ -130,6 +130,14 @@ __device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3
 #pragma unroll
     for (int i = 0; i < 3; i++) uh[r][i] = ph * uh[r][i];
   recon_acc<MU, S>(acc, uh, coef);
+}
+
+template <int MU, int S, bool FWD, bool C12, typename T>
+__device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3], T coef, T tr, T ti,
+                                 cx<T> (&acc)[4][3]) {
+  cx<T> h[2][3];
+  project<MU, S>(psi, h);
+  apply_hop_h<MU, S, FWD, C12>(U, h, coef, tr, ti, acc);
 }
 
 template <int P, bool C12, int ISIGN, int MU>
@@ -167,6 +175,64 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
     make_link<C12>(lv + GL::link_reals, U);
     apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr, ti, acc);
   }
+}
+
+// Direction MU of the hop with the neighbour half spinors taken from the
+// lanes of the same wavefront where possible. When the wave's 64 target sites
+// are consecutive (s0 .. s0 + 63), the source spinor at index n is the
+// wave's own-index spinor of lane n - s0, so each lane projects its own-index
+// spinor with the direction's projector and the consumers read the 12-real
+// half spinor across lanes; lanes whose neighbour lies outside the wave load
+// it. The arithmetic is that of hop_dim (same projection, same products).
+template <typename T> __device__ inline void shfl_half(const cx<T> (&ho)[2][3], int src, cx<T> (&h)[2][3]) {
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) h[r][c] = mk(__shfl(ho[r][c].r, src, 64), __shfl(ho[r][c].i, src, 64));
+}
+
+template <int P, bool C12, int ISIGN, int MU, bool FWD>
+__device__ inline void hop_half_shfl(const HopArgs &a, const void *psi_p, const cx<typename PT<P>::calc> (&own)[4][3],
+                                     unsigned s, unsigned sn, unsigned s0, bool contig, typename PT<P>::calc coef,
+                                     typename PT<P>::calc tr, typename PT<P>::calc ti,
+                                     cx<typename PT<P>::calc> (&acc)[4][3]) {
+  typedef typename PT<P>::calc T;
+  typedef GaugeLayout<P, C12> GL;
+  constexpr int S = FWD ? -ISIGN : ISIGN;
+  cx<T> ho[2][3], h[2][3];
+  project<MU, S>(own, ho);
+  const unsigned d = sn - s0;
+  const bool in = contig && d < 64u;
+  shfl_half(ho, in ? (int)d : 0, h);
+  if (!in) {
+    T sp[kSpinorReals];
+    cx<T> ps[4][3];
+    load_spinor<P>(psi_p, a.vs, sn, sp);
+    to_spinor(sp, ps);
+    project<MU, S>(ps, h);
+  }
+  cx<T> U[3][3];
+  if constexpr (GL::per_link) {
+    T lv[GL::link_reals];
+    load_range<P, GL::nw, (2 * MU + (FWD ? 1 : 0)) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv, U);
+  } else {
+    T lv[2 * GL::link_reals];
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
+    make_link<C12>(lv + (FWD ? GL::link_reals : 0), U);
+  }
+  apply_hop_h<MU, S, FWD, C12>(U, h, coef, tr, ti, acc);
+}
+
+template <int P, bool C12, int ISIGN, int MU>
+__device__ inline void hop_dim_shfl(const HopArgs &a, const void *psi_p, const cx<typename PT<P>::calc> (&own)[4][3],
+                                    unsigned s, unsigned sb, unsigned sf, unsigned s0, bool contig,
+                                    typename PT<P>::calc coef, cx<typename PT<P>::calc> (&acc)[4][3]) {
+  typedef typename PT<P>::calc T;
+  const bool tbc = (a.tbc_mask >> MU) & 1;
+  const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
+  hop_half_shfl<P, C12, ISIGN, MU, false>(a, psi_p, own, s, sb, s0, contig, coef, tr, ti, acc);
+  hop_half_shfl<P, C12, ISIGN, MU, true>(a, psi_p, own, s, sf, s0, contig, coef, tr, ti, acc);
 }
 
 // --- clover blocks --------------------------------------------------------
@@ -505,9 +571,19 @@ __device__ inline void hop_finish_r(const HopArgs &a, unsigned s, const cx<typen
 }
 
 // the hop for traversal index k (< a.vh) of R right-hand sides psi[j] -> out[j]
+// Kernels that take their x / y neighbours across lanes (hop_dim_shfl). The
+// exchange keeps the own-index spinor live through two dimensions, which costs
+// registers, and the 24 lane exchanges per site cost the same whatever the
+// storage: measured at 48^3x96 it makes the FP32 forms ~5% faster, the FP16
+// forms (52-byte spinors, cheap to load) ~8% slower, and the FP64 forms would
+// drop to one or two waves per SIMD. FP32 only.
+constexpr bool hop_shfl(int p, int kind) { return p == P32 && kind != KD_DIR; }
+
+// wave_full: all 64 lanes of the wavefront run this site loop (the x / y
+// neighbours may then come from other lanes, hop_dim_shfl)
 template <int P, bool C12, int ISIGN, int KIND, int R>
 __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *const (&psi)[R], void *const (&out)[R],
-                                  const void *const (&chi)[R]) {
+                                  const void *const (&chi)[R], bool wave_full = false) {
   typedef typename PT<P>::calc T;
   const SiteNbrs n = site_nbrs(a, k);
   const unsigned s = n.s;
@@ -529,8 +605,19 @@ __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *cons
       default: hop_term_r<P, C12, ISIGN, 3, false, R>(a, psi, s, n.tb, ctb, acc); break;
     }
   } else if constexpr (R == 1) {
-    hop_dim<P, C12, ISIGN, 0>(a, psi[0], s, n.xb, n.xf, cs, cs, acc[0]);
-    hop_dim<P, C12, ISIGN, 1>(a, psi[0], s, n.yb, n.yf, cs, cs, acc[0]);
+    if (hop_shfl(P, KIND) && wave_full) {
+      T ov[kSpinorReals];
+      cx<T> own[4][3];
+      load_spinor<P>(psi[0], a.vs, s, ov);
+      to_spinor(ov, own);
+      const unsigned s0 = __shfl(s, 0, 64);
+      const bool contig = __shfl(s, 63, 64) == s0 + 63u;
+      hop_dim_shfl<P, C12, ISIGN, 0>(a, psi[0], own, s, n.xb, n.xf, s0, contig, cs, acc[0]);
+      hop_dim_shfl<P, C12, ISIGN, 1>(a, psi[0], own, s, n.yb, n.yf, s0, contig, cs, acc[0]);
+    } else {
+      hop_dim<P, C12, ISIGN, 0>(a, psi[0], s, n.xb, n.xf, cs, cs, acc[0]);
+      hop_dim<P, C12, ISIGN, 1>(a, psi[0], s, n.yb, n.yf, cs, cs, acc[0]);
+    }
     hop_dim<P, C12, ISIGN, 2>(a, psi[0], s, n.zb, n.zf, cs, cs, acc[0]);
     hop_dim<P, C12, ISIGN, 3>(a, psi[0], s, n.tb, n.tf, ctb, ctf, acc[0]);
   } else {
@@ -544,11 +631,12 @@ __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *cons
 
 // the hop for traversal index k (< a.vh) of one right-hand side
 template <int P, bool C12, int ISIGN, int KIND>
-__device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, void *out_p, const void *chi_p) {
+__device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, void *out_p, const void *chi_p,
+                                bool wave_full = false) {
   const void *const psi[1] = {psi_p};
   void *const out[1] = {out_p};
   const void *const chi[1] = {chi_p};
-  hop_body_r<P, C12, ISIGN, KIND, 1>(a, (unsigned)k, psi, out, chi);
+  hop_body_r<P, C12, ISIGN, KIND, 1>(a, (unsigned)k, psi, out, chi, wave_full);
 }
 
 // minimum waves per SIMD the register allocation must allow for the hop
@@ -577,8 +665,11 @@ __global__ void __launch_bounds__(256, hop_min_waves(P)) hop_kernel(const HopArg
     chi_p = ka->chis[r];
   }
   const long k = (long)lb * blockDim.x + threadIdx.x;
+  // the cross-lane neighbour exchange needs every lane of the wave (one
+  // RHS per launch: the per-RHS pointers are uniform over the wave)
+  const bool wave_full = __ballot(k < a.vh) == ~0ull;
   if (k >= a.vh) return;
-  hop_body<P, C12, ISIGN, KIND>(a, k, psi_p, out_p, chi_p);
+  hop_body<P, C12, ISIGN, KIND>(a, k, psi_p, out_p, chi_p, wave_full);
 }
 
 // Multi-RHS hop with a register tile of R right-hand sides per work-item:
