@@ -338,10 +338,10 @@ int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[]
  *      over all unconverged systems; scalars stay on the device in FP64; with
  *      graph_steps > 0 one CG iteration is captured in a HIP graph and replayed
  *      graph_steps times between host convergence checks. Systems given in
- *      groups of four consecutive entries with the same operator (four
- *      right-hand sides per lattice) run their hops as a register tile when
- *      every lattice has at least 2^17 sites per parity (QPHIX_BATCH_TILE=0 /
- *      1 forces it off / on). */
+ *      pairs of consecutive entries with the same operator (several
+ *      right-hand sides per lattice) run their hops as a register tile of two
+ *      sources when the batch has at least 2^18 sites per parity in all
+ *      (QPHIX_BATCH_TILE=0 / 1 forces it off / on). */
 typedef struct qphix_batch qphix_batch;
 int qphix_batch_create(qphix_op *const ops[], int nsys, qphix_batch **b);
 int qphix_batch_destroy(qphix_batch *b);

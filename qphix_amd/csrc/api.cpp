@@ -2481,14 +2481,16 @@ extern "C" int qphix_batch_create(qphix_op *const ops[], int nsys, qphix_batch *
   for (int i = 0; i < nsys; i++) b->off[i + 1] = b->off[i] + (int)((ops[i]->g->d.vh + 255) / 256);
   // Systems that come in groups of kBatchTile sharing one operator (several
   // right-hand sides per lattice) can run their hops as a register tile, which
-  // reads links and clover once per group. That pays where the hops are
-  // bandwidth-bound; on small lattices the batch is latency-bound and one
-  // source per work-item (4x the work-items) is faster. Default: tile when
-  // every lattice has at least 2^17 sites per parity; QPHIX_BATCH_TILE=0 / 1
-  // forces it off / on (where the grouping allows).
+  // reads links and clover once per group. That pays where the batched hops
+  // are bandwidth-bound, i.e. when the batch holds enough sites; a batch of a
+  // few small lattices is latency-bound, and one source per work-item (twice
+  // the work-items) is faster. Default: tile when the batch has at least 2^18
+  // sites per parity in all; QPHIX_BATCH_TILE=0 / 1 forces it off / on (where
+  // the grouping allows).
   const char *te = getenv("QPHIX_BATCH_TILE");
-  bool want = true;
-  for (int i = 0; i < nsys; i++) want = want && ops[i]->g->d.vh >= (1L << 17);
+  long sites = 0;
+  for (int i = 0; i < nsys; i++) sites += ops[i]->g->d.vh;
+  bool want = sites >= (1L << 18);
   if (te && te[0] == '0') want = false;
   if (te && te[0] == '1') want = true;
   b->tile = want && nsys % kBatchTile == 0 ? kBatchTile : 1;

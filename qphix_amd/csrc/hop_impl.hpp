@@ -704,7 +704,7 @@ __global__ void __launch_bounds__(256) hop_tile_kernel(const HopArgs a) {
 // form), so the launch shape does not depend on the batch and can be captured
 // once in a graph. Segments whose system has converged (active[sys] == 0) are
 // skipped.
-constexpr int kBatchTile = 4;  // systems per segment of the tiled batched hop
+constexpr int kBatchTile = 2;  // systems per segment of the tiled batched hop
 struct HopSeg {
   HopArgs a;
   int sys;   // first system of the segment

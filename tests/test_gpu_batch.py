@@ -95,10 +95,10 @@ def force_batch_tile():
 
 
 def test_batch_register_tile_matches_untiled(systems, force_batch_tile):
-    """Four right-hand sides per lattice, grouped (each group of four systems
-    shares its operator, so the hops run as a register tile of four sources)
-    and interleaved (no group shares an operator: one source per work-item):
-    both orders give bit-identical solutions, and the true residuals hold."""
+    """Four right-hand sides per lattice, grouped (each pair of systems shares
+    its operator, so the hops run as a register tile of two sources) and
+    interleaved (no pair shares an operator: one source per work-item): both
+    orders give bit-identical solutions, and the true residuals hold."""
     n_rhs = 4
     grouped = [(li, r) for li in range(len(LATS)) for r in range(n_rhs)]
     inter = [(li, r) for r in range(n_rhs) for li in range(len(LATS))]
