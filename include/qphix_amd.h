@@ -349,7 +349,9 @@ int qphix_batch_destroy(qphix_batch *b);
 int qphix_batch_cg(qphix_batch *b, void *const x[], const void *const rhs[], const double rsd_target[],
                    int max_iters, int isign, int target_cb, int graph_steps, qphix_solve_stats st[]);
 /* FP64 defect correction (outer[i] on the same lattice as the batch's
- * operator i) around the batched inner CG on M^+M, relaxation delta */
+ * operator i) around the batched inner CG on M^+M, relaxation delta. The
+ * outer loop's work spinors live in the batch object from the first call to
+ * qphix_batch_destroy, so calls on one batch must not overlap. */
 int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[], void *const x[],
                            const void *const rhs[], double rsd_target, double delta, int max_outer,
                            int max_inner, int isign, int target_cb, int graph_steps,

@@ -2422,6 +2422,10 @@ struct qphix_batch {
   std::vector<qphix_op *> ops;
   int n = 0, prec = 0, c12 = 0, kind = 0;
   std::vector<void *> tmp, mp, mmp, p, r;  // per-system work spinors
+  // mixed-precision Richardson work spinors, kept between calls while the
+  // outer spinor sizes stay the same: r, dx, tmp, tmp2 (outer), ri, dxi (inner)
+  std::vector<size_t> rich_bytes;
+  std::vector<void *> rich[6];
   std::vector<int> off;                    // chunk / work-group offsets (256 sites)
   int *d_off = nullptr, *d_active = nullptr;
   // hop segments: one per system, or (tile > 1) one per group of `tile`
@@ -2450,6 +2454,8 @@ extern "C" int qphix_batch_destroy(qphix_batch *b) {
   std::vector<void *> *sets[5] = {&b->tmp, &b->mp, &b->mmp, &b->p, &b->r};
   for (auto *v : sets)
     for (void *q : *v) qphix_free(q);
+  for (auto &v : b->rich)
+    for (void *q : v) qphix_free(q);
   batch_free_dev(b);
   delete b;
   return QPHIX_OK;
@@ -2770,22 +2776,24 @@ extern "C" int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[
     CHKPTR(outer[i]);
     TRY(same_lattice(outer[i]->g, inner->ops[i]->g));
   }
-  std::vector<void *> r(n), dx(n), tmp(n), tmp2(n), ri(n), dxi(n);
-  struct Guard {
-    std::vector<void *> *v[6];
-    ~Guard() {
-      for (auto *q : v)
-        for (void *p : *q) qphix_free(p);
+  // work spinors: allocated on the first call (or when an outer spinor size
+  // changed) and kept in the batch object until qphix_batch_destroy
+  bool reuse = (int)inner->rich_bytes.size() == n;
+  for (int i = 0; reuse && i < n; i++) reuse = inner->rich_bytes[i] == outer[i]->g->bytes[0];
+  if (!reuse) {
+    for (auto &v : inner->rich) {
+      for (void *q : v) qphix_free(q);
+      v.assign(n, nullptr);
     }
-  } guard{{&r, &dx, &tmp, &tmp2, &ri, &dxi}};
-  for (int i = 0; i < n; i++) {
-    TRY(qphix_alloc_spinor(outer[i]->g, &r[i]));
-    TRY(qphix_alloc_spinor(outer[i]->g, &dx[i]));
-    TRY(qphix_alloc_spinor(outer[i]->g, &tmp[i]));
-    TRY(qphix_alloc_spinor(outer[i]->g, &tmp2[i]));
-    TRY(qphix_alloc_spinor(inner->ops[i]->g, &ri[i]));
-    TRY(qphix_alloc_spinor(inner->ops[i]->g, &dxi[i]));
+    inner->rich_bytes.assign(n, 0);
+    for (int i = 0; i < n; i++) {
+      for (int k = 0; k < 6; k++)
+        TRY(qphix_alloc_spinor(k < 4 ? outer[i]->g : inner->ops[i]->g, &inner->rich[k][i]));
+      inner->rich_bytes[i] = outer[i]->g->bytes[0];
+    }
   }
+  const std::vector<void *> &r = inner->rich[0], &dx = inner->rich[1], &tmp = inner->rich[2],
+                            &tmp2 = inner->rich[3], &ri = inner->rich[4], &dxi = inner->rich[5];
   std::vector<double> rhs_sq(n), rn2(n), rsd_t(n), tgt(n);
   std::vector<int> outer_it(n, 0);
   std::vector<unsigned long> mv(n, 0);
