@@ -6,4 +6,5 @@
 #include "solvers.h"
 #include "full_spinor.h"
 #include "unprec_solver_wrapper.h"
+#include "syssolver_linop_clover.h"
 #include "blas_new_c.h"

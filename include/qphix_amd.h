@@ -396,6 +396,84 @@ int qphix_bicgstab_rxupdate(const qphix_geom *g, void *x, void *r, const void *t
 int qphix_mr_rxupdate(const qphix_geom *g, void *x, void *r, const void *mr,
                       const double a[2]);                                                /* 796 */
 
+/* ---- Chroma LinOpSysSolver surface (README.markdown:163-256): the
+ *      <InvertParam> group of QPHIX_CLOVER_INVERTER and
+ *      QPHIX_CLOVER_ITER_REFINE_BICGSTAB_INVERTER mapped onto this library.
+ *      The clover term follows tests/clover_term_qdp_w.h:156-180:
+ *        spatial hop coefficient  aniso_s = nu / xi_0 (anisoP), else 1
+ *        temporal hop coefficient aniso_t = 1, t_boundary = -1 if AntiPeriodicT
+ *        csw_r = clovCoeffR / xi_0 (anisoP), csw_t = clovCoeffT
+ *        diag_mass = 1 + 3 aniso_s + Mass, Mass = 1/(2 Kappa) - 4 from Kappa
+ *      and the system solved is the even-odd clover operator on the odd
+ *      checkerboard, A_oo - 1/4 D_oe A_ee^-1 D_eo (clover.h:162-172). */
+enum { QPHIX_INV_CLOVER = 0, QPHIX_INV_CLOVER_ITER_REFINE = 1 };
+enum { QPHIX_SOLVER_CG = 0, QPHIX_SOLVER_BICGSTAB = 1 };
+typedef struct {
+  int inv_type;              /* <invType>                                         */
+  int solver_type;           /* <SolverType>: CG solves M^+M psi = M^+ chi          */
+  int max_iter;              /* <MaxIter> (outer and inner for iterative refinement) */
+  double rsd_target;         /* <RsdTarget>, relative                             */
+  double delta;              /* <Delta>: inner residual reduction (refinement)    */
+  double mass;               /* <Mass>, or from <Kappa>                           */
+  double clov_coeff_r;       /* <clovCoeffR> (or <clovCoeff>)                     */
+  double clov_coeff_t;       /* <clovCoeffT> (or <clovCoeff>)                     */
+  int aniso_p;               /* <AnisoParam><anisoP>                              */
+  int t_dir;                 /* <t_dir>: 3 is the only direction supported        */
+  double xi_0, nu;           /* <xi_0>, <nu>                                      */
+  int antiperiodic_t;        /* <AntiPeriodicT>                                   */
+  int verbose;               /* <Verbose>: one summary line per solve on stderr   */
+  int tune;                  /* <Tune>: autotune the operator's kernels at create */
+  double rsd_tolerance_factor; /* <RsdToleranceFactor>: a solve whose true relative */
+                               /* residual exceeds RsdTarget times this fails       */
+  /* <NCores> <ThreadsPerCore> <By> <Bz> <Sy> <Sz> <PadXY> <PadXYZ> <MinCt>:
+   * host threading and cache blocking of the reference; read and checked,
+   * without effect here (the hop kernels are autotuned instead) */
+  int ncores, threads_per_core, by, bz, sy, sz, pad_xy, pad_xyz, min_ct;
+  /* precisions, configure options of the reference build
+   * (--enable-qphix-solver-compress12, --enable-qphix-solver-inner-type);
+   * XML: <OuterPrecision> d|f, <InnerPrecision> h|f|d, <Compress12> (both),
+   * <InnerCompress12> (inner only; half precision needs it) */
+  int prec, compress12, inner_prec, inner_compress12;
+} qphix_invert_param;
+/* defaults: BiCGStab, MaxIter 1000, RsdTarget 1e-7, Delta 0.1, isotropic,
+ * c_sw 1, periodic t, RsdToleranceFactor 5, outer F64, inner F32, no
+ * compression */
+int qphix_invert_param_default(qphix_invert_param *p);
+/* reads the first <InvertParam> element of an XML document into p (starting
+ * from the defaults). Malformed XML and missing or unreadable tags:
+ * QPHIX_EINVAL; unsupported values (invType, t_dir, both Mass and Kappa, ...):
+ * QPHIX_EDOMAIN. qphix_last_error names the tag and line. */
+int qphix_invert_param_read_xml(const char *xml, qphix_invert_param *p);
+/* the coefficients above: out = {aniso_s, aniso_t, t_boundary, csw_r, csw_t,
+ * diag_mass} (host only) */
+int qphix_invert_param_coeffs(const qphix_invert_param *p, double out[6]);
+
+typedef struct qphix_syssolver qphix_syssolver;
+typedef struct {
+  int n_count;               /* solver iterations (outer ones for refinement)     */
+  double resid;              /* true |chi - M psi| on the odd checkerboard        */
+  double rel_resid;          /* resid / |chi|                                     */
+  unsigned long mv_apps;     /* operator applications incl. the residual check    */
+  double seconds;            /* wall time of the solve                            */
+  int status;                /* QPHIX_OK, QPHIX_ENOCONV, QPHIX_EBREAKDOWN         */
+} qphix_syssolver_results;
+/* builds the gauge field (canonical host layout above), the clover term A on
+ * the odd and A^-1 on the even checkerboard in FP64 on the device, the
+ * operators and the solver(s) */
+int qphix_syssolver_create(const int latt[4], const qphix_invert_param *p, const double *u,
+                           qphix_syssolver **s);
+int qphix_syssolver_destroy(qphix_syssolver *s);
+/* psi, chi: canonical full-lattice spinors (double[2*Vh][4][3][2]); the odd
+ * half of psi is the initial guess and receives the solution, the even half is
+ * not touched. The call returns QPHIX_OK when the solve ran; r->status tells
+ * whether it met RsdTarget * RsdToleranceFactor. */
+int qphix_syssolver_solve(qphix_syssolver *s, double *psi, const double *chi,
+                          qphix_syssolver_results *r);
+/* the same on device spinors of the outer geometry (odd checkerboard) */
+int qphix_syssolver_solve_dev(qphix_syssolver *s, void *psi, const void *chi,
+                              qphix_syssolver_results *r);
+int qphix_syssolver_geom(const qphix_syssolver *s, const qphix_geom **g);
+
 #ifdef __cplusplus
 }
 #endif

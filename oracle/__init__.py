@@ -155,10 +155,14 @@ def achimbdpsi(L, u, psi, chi, alpha, beta, isign, cb, p=None):
     return out
 
 
-def clover_build(L, u, csw, mass):
-    """A = (4+mass) + clover term, both parities, cb-major (72 per site)."""
+def clover_build(L, u, csw, mass, csw_t=None, diag_mass=None):
+    """A = (4+mass) + clover term, both parities, cb-major (72 per site).
+    csw weighs the planes without t and, unless csw_t is given, those with t;
+    diag_mass replaces 4+mass when given (orc_clover_build)."""
     clov = np.empty(2 * vh(L) * 72)
-    lib().orc_clover_build(I4(*L), ptr(u), csw, csw, 4.0 + mass, ptr(clov))
+    ct = csw if csw_t is None else csw_t
+    dm = 4.0 + mass if diag_mass is None else diag_mass
+    lib().orc_clover_build(I4(*L), ptr(u), csw, ct, dm, ptr(clov))
     return clov
 
 

@@ -217,11 +217,16 @@ def gen_spinor(geom, seed, cb):
     return s
 
 
-def clover_build(geom, u, csw, mass, cb, what=CLOV_A, mu=0.0, eps=0.0):
+def clover_build(geom, u, csw, mass, cb, what=CLOV_A, mu=0.0, eps=0.0, csw_t=None,
+                 diag_mass=None):
+    """csw is the spatial-plane coefficient (and the temporal one unless csw_t
+    is given); the diagonal is 4 + mass unless diag_mass is given."""
     full = what in (CLOV_TMINV, CLOV_ND_UP, CLOV_ND_DN)
     f = geom.full_clover() if full else geom.clover()
-    call("qphix_clover_build", geom.h, u[0].p, u[1].p, C.c_double(csw), C.c_double(csw),
-         C.c_double(4.0 + mass), cb, what, C.c_double(mu), C.c_double(eps), f.p)
+    ct = csw if csw_t is None else csw_t
+    dm = 4.0 + mass if diag_mass is None else diag_mass
+    call("qphix_clover_build", geom.h, u[0].p, u[1].p, C.c_double(csw), C.c_double(ct),
+         C.c_double(dm), cb, what, C.c_double(mu), C.c_double(eps), f.p)
     return f
 
 
@@ -587,3 +592,92 @@ def copy(geom, res, src):
 
 def zero(geom, res):
     call("qphix_zero_spinor", geom.h, res.p)
+
+
+# ---------------------------------------------------------------------------
+# Chroma LinOpSysSolver surface (README.markdown:163-256)
+INV_CLOVER, INV_CLOVER_ITER_REFINE = 0, 1
+SOLVER_CG, SOLVER_BICGSTAB = 0, 1
+
+
+class InvertParam(C.Structure):
+    """The <InvertParam> group of the QPhiX clover inverters."""
+    _fields_ = [("inv_type", C.c_int), ("solver_type", C.c_int), ("max_iter", C.c_int),
+                ("rsd_target", C.c_double), ("delta", C.c_double), ("mass", C.c_double),
+                ("clov_coeff_r", C.c_double), ("clov_coeff_t", C.c_double),
+                ("aniso_p", C.c_int), ("t_dir", C.c_int), ("xi_0", C.c_double),
+                ("nu", C.c_double), ("antiperiodic_t", C.c_int), ("verbose", C.c_int),
+                ("tune", C.c_int), ("rsd_tolerance_factor", C.c_double),
+                ("ncores", C.c_int), ("threads_per_core", C.c_int), ("by", C.c_int),
+                ("bz", C.c_int), ("sy", C.c_int), ("sz", C.c_int), ("pad_xy", C.c_int),
+                ("pad_xyz", C.c_int), ("min_ct", C.c_int), ("prec", C.c_int),
+                ("compress12", C.c_int), ("inner_prec", C.c_int),
+                ("inner_compress12", C.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        p = cls()
+        call("qphix_invert_param_default", C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        return p
+
+    @classmethod
+    def from_xml(cls, xml):
+        p = cls()
+        call("qphix_invert_param_read_xml", xml.encode() if isinstance(xml, str) else xml,
+             C.byref(p))
+        return p
+
+    def coeffs(self):
+        """(aniso_s, aniso_t, t_boundary, csw_r, csw_t, diag_mass)"""
+        out = (C.c_double * 6)()
+        call("qphix_invert_param_coeffs", C.byref(self), out)
+        return tuple(out)
+
+
+class SysSolverResults(C.Structure):
+    _fields_ = [("n_count", C.c_int), ("resid", C.c_double), ("rel_resid", C.c_double),
+                ("mv_apps", C.c_ulong), ("seconds", C.c_double), ("status", C.c_int)]
+
+
+class SysSolver:
+    """The odd-checkerboard clover system M psi = chi described by an
+    InvertParam, on a canonical host gauge field."""
+
+    def __init__(self, latt, param, u_canonical):
+        if isinstance(param, (str, bytes)):
+            param = InvertParam.from_xml(param)
+        self.param = param
+        self.latt = list(latt)
+        a = np.ascontiguousarray(u_canonical, dtype=np.float64)
+        h = VP()
+        call("qphix_syssolver_create", (C.c_int * 4)(*latt), C.byref(param), dptr(a), C.byref(h))
+        self.h = h
+        g = VP()
+        call("qphix_syssolver_geom", h, C.byref(g))
+        vh = C.c_long()
+        call("qphix_geom_info", g, None, None, None, C.byref(vh), None)
+        self.vh = vh.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().qphix_syssolver_destroy(self.h)
+        except Exception:
+            pass
+
+    def __call__(self, psi, chi):
+        """psi (updated in place), chi: canonical full-lattice spinors."""
+        assert psi.size == chi.size == 2 * self.vh * 24
+        c = np.ascontiguousarray(chi, dtype=np.float64)
+        r = SysSolverResults()
+        call("qphix_syssolver_solve", self.h, dptr(psi), dptr(c), C.byref(r))
+        return r
+
+    def solve_dev(self, psi, chi):
+        r = SysSolverResults()
+        call("qphix_syssolver_solve_dev", self.h, psi.p, chi.p, C.byref(r))
+        return r

@@ -105,6 +105,9 @@ static int fail(int code, const char *fmt, ...) {
   g_err = buf;
   return code;
 }
+// the error setter of this file for the other host translation units
+int qphix_amd::api_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+
 #define HIPCHK(call)                                                                              \
   do {                                                                                            \
     hipError_t e_ = (call);                                                                       \

@@ -84,4 +84,7 @@ struct MultiBlasArgs {
 hipError_t blas_multi_launch(const GeomDesc &g, int op, const MultiBlasArgs &m, double *partials, double *out,
                              hipStream_t st);
 
+// records msg as the thread's last error (qphix_last_error) and returns code
+int api_fail(int code, const char *msg);
+
 }  // namespace qphix_amd

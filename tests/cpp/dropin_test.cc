@@ -5,7 +5,9 @@
 // on the first failure.
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "qphix/qphix.h"
@@ -202,6 +204,48 @@ int main()
     double n;
     xmyNorm2Spinor(r, b, t, n, geom, 1);
     check("mr_true_resid", std::sqrt(n / bnorm), 1.5e-9);
+  }
+
+  // the Chroma-facing system solver from an <InvertParam> group on unit links:
+  // a constant odd-site source is an eigenvector of the even-odd operator with
+  // eigenvalue m (8 + m) / (4 + m)
+  {
+    const long vh = 8 * 8 * 8 * 8 / 2;
+    std::vector<double> unit(2 * vh * 4 * 18, 0.0), chi(2 * vh * 24, 0.0), psi(2 * vh * 24, 0.0);
+    for (long i = 0; i < 2 * vh * 4; i++)
+      for (int c = 0; c < 3; c++) unit[i * 18 + c * 8] = 1.0;
+    for (long i = vh * 24; i < 2 * vh * 24; i++) chi[i] = 0.25 + 0.01 * (i % 24);
+    const std::string xml =
+        "<InvertParam><invType>QPHIX_CLOVER_ITER_REFINE_BICGSTAB_INVERTER</invType>"
+        "<SolverType>BICGSTAB</SolverType><MaxIter>100</MaxIter><RsdTarget>1e-10</RsdTarget>"
+        "<Delta>0.1</Delta><CloverParams><Mass>0.1</Mass><clovCoeff>1.0</clovCoeff></CloverParams>"
+        "<NCores>8</NCores><Tune>true</Tune></InvertParam>";
+    CloverLinOpSysSolver solve(latt, xml, unit.data());
+    SystemSolverResults res = solve(psi.data(), chi.data());
+    const double lam = 0.1 * 8.1 / 4.1;
+    double err = 0, mx = 0;
+    for (long i = vh * 24; i < 2 * vh * 24; i++) {
+      err = std::fmax(err, std::fabs(psi[i] - chi[i] / lam));
+      mx = std::fmax(mx, std::fabs(chi[i] / lam));
+    }
+    check("syssolver_free_field", err / mx, 1e-9);
+    check("syssolver_rel_resid", res.rel_resid, 5e-10);
+    // a solve that cannot reach its target throws, as the reference's caller aborts
+    qphix_invert_param p;
+    detail::check(qphix_invert_param_read_xml(xml.c_str(), &p), "InvertParam");
+    p.inv_type = QPHIX_INV_CLOVER;
+    p.max_iter = 1;
+    p.rsd_target = 1e-14;
+    CloverLinOpSysSolver fails(latt, p, unit.data());
+    for (long i = vh * 24; i < 2 * vh * 24; i++) chi[i] = (i * 7919 % 101) * 0.01;
+    std::fill(psi.begin(), psi.end(), 0.0);
+    bool threw = false;
+    try {
+      fails(psi.data(), chi.data());
+    } catch (const std::runtime_error &) {
+      threw = true;
+    }
+    check("syssolver_failed_solve_throws", threw ? 0 : 1, 0);
   }
 
   for (int cb = 0; cb < 2; cb++) {

@@ -236,6 +236,24 @@ def test_gpu_clover_build_matches_oracle(c12):
             assert rel(out.download(cb), oracle.full_clover_apply(ref, xh)) < 1e-12
 
 
+def test_gpu_anisotropic_clover_build_matches_oracle():
+    """Separate spatial / temporal clover coefficients and a free diagonal
+    (the anisotropic clover term of tests/clover_term_qdp_w.h:156-180)."""
+    L = [4, 4, 4, 8]
+    vh = 256
+    g = q.Geometry(L, q.F64, False)
+    u = oracle.gen_gauge(L, 10, 0.4)
+    ud = q.pack_gauge(g, u)
+    cr, ct, dm = 0.7, 1.3, 2.2
+    clov = oracle.clover_build(L, u, cr, 0.0, csw_t=ct, diag_mass=dm)
+    for cb in (0, 1):
+        ch = clov[cb * vh * 72:(cb + 1) * vh * 72]
+        a = q.clover_build(g, ud, cr, 0.0, cb, q.CLOV_A, csw_t=ct, diag_mass=dm)
+        assert rel(a.download(), ch) < 1e-13
+        ai = q.clover_build(g, ud, cr, 0.0, cb, q.CLOV_AINV, csw_t=ct, diag_mass=dm)
+        assert rel(ai.download(), oracle.clover_invert(ch)) < 1e-12
+
+
 def test_precision_conversion():
     L = [4, 4, 4, 8]
     g64 = q.Geometry(L, q.F64, False)

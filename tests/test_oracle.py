@@ -356,6 +356,38 @@ def test_clover_sigma_f_structure(orc):
     assert abs(coef[0] + 0.5 * csw) < 1e-12
 
 
+def test_clover_plane_coefficients(orc):
+    """Anisotropic clover term: csw_r weighs the planes without t and csw_t the
+    planes with t (clovCoeffR / clovCoeffT, tests/clover_term_qdp_w.h:156-166),
+    and diag_mass is the whole diagonal. The sigma (x) F fit of the isotropic
+    test, with one free coefficient per plane, must return -csw/2 of the
+    plane's kind on each."""
+    L = [4, 4, 4, 4]
+    u = orc.gen_gauge(L, 9, 0.4)
+    cr, ct, dm = 1.25, 0.5, 3.0
+    clov = orc.clover_build(L, u, cr, 0.0, csw_t=ct, diag_mass=dm)
+    f = np.empty(2 * 128 * 6 * 18)
+    orc.lib().orc_field_strength((orc.C.c_int * 4)(*L), orc.ptr(u), orc.ptr(f))
+    F = cplx(f).reshape(-1, 6, 3, 3)
+    sig = pauli_sigma(gammas(orc))
+    planes = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    rows, rhs = [], []
+    for i in range(0, 256, 17):
+        A = np.zeros((12, 12), dtype=np.complex128)
+        for b in range(2):
+            A[6 * b:6 * b + 6, 6 * b:6 * b + 6] = orc.clover_dense(clov.reshape(-1, 72)[i], b)
+        A -= dm * np.eye(12)
+        rows.append(np.stack([np.kron(sig[pl], F[i, k]).ravel()
+                              for k, pl in enumerate(planes)], axis=1))
+        rhs.append(A.ravel())
+    M = np.concatenate(rows)
+    y = np.concatenate(rhs)
+    coef, *_ = np.linalg.lstsq(M, y, rcond=None)
+    assert np.abs(M @ coef - y).max() < 1e-12
+    want = [-0.5 * (ct if 3 in pl else cr) for pl in planes]
+    assert np.allclose(coef, want, atol=1e-12), coef
+
+
 def test_clover_covariance_and_inverse(orc):
     L = [4, 4, 4, 4]
     rng = np.random.default_rng(2)
