@@ -558,6 +558,13 @@ def norm2(geom, x):
     return r.value
 
 
+def xmy_norm2(geom, res, x, y):
+    """res = x - y; returns |res|^2 (xmyNorm2Spinor)."""
+    r = C.c_double()
+    call("qphix_xmy_norm2", geom.h, res.p, x.p, y.p, C.byref(r))
+    return r.value
+
+
 def block_gram(geom, a, b):
     """G[i, j] = <a_i, b_j> (complex numpy matrix)."""
     o = np.zeros(len(a) * len(b) * 2)
