@@ -9,6 +9,9 @@
  * half-spinor projection), which is a different formulation from the GPU
  * kernels (projector / reconstruct); both restate tests/dslashm_w.cc:45-144.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "qphix_oracle.h"
 
 #include <math.h>
@@ -264,6 +267,8 @@ void orc_hop_masked(const int L[4], const double *u, const double *psi, double *
 {
   long vh = orc_vh(L);
   int ocb = 1 - cb;
+  /* sites are independent: the threads share no writes */
+#pragma omp parallel for schedule(static)
   for (long j = 0; j < vh; j++) {
     int c[4];
     orc_site_coords(L, cb * vh + j, c);
@@ -606,6 +611,7 @@ void orc_ndclover_inverse(long nsites, const double *clov, double mu, double eps
  * 2b, 2b+1 with index 3*(spin-2b)+colour */
 void orc_clover_apply(long nsites, const double *clov, const double *in, double *out)
 {
+#pragma omp parallel for schedule(static)
   for (long i = 0; i < nsites; i++)
     for (int b = 0; b < 2; b++) {
       double M[72];
@@ -1004,4 +1010,14 @@ void orc_minvcg(const orc_op *op, double *x, const double *rhs, int n_shift,
   }
   free(r); free(p0); free(mp); free(mmp); free(p); free(zeta); free(zeta_prev);
   free(beta); free(rsd_sq); free(conv);
+}
+
+/* threads the parallel site loops use (1 without OpenMP) */
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

@@ -131,6 +131,9 @@ void orc_minvcg(const orc_op *op, double *x, const double *rhs, int n_shift,
                 const double *shifts, const double *rsd_target, int max_iters,
                 int isign, int cb, int *n_iters, double *rsd_sq_final);
 
+/* threads of the OpenMP site loops (hop, clover apply); 1 without OpenMP */
+int orc_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif
