@@ -2,10 +2,11 @@
  * qphix_oracle.h -- CPU oracle for the even-odd Wilson / clover / twisted-mass
  * Dslash path, its fused BLAS and its Krylov solvers.
  *
- * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, single-threaded
- * restatement of the QPhiX algorithms that provides the reference results of
- * tests/ and of the smoke check, and the CPU baseline leg of bench.py.  The product path
- * (libqphix_amd.so) never calls it.
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64 restatement of the
+ * QPhiX algorithms (site loops of the hop and the clover apply split over
+ * OpenMP threads, each site computed alone) that provides the reference
+ * results of tests/ and of the smoke check, and the CPU baseline leg of
+ * bench.py.  The product path (libqphix_amd.so) never calls it.
  *
  * Host layout (the "canonical" layout, the same one the C ABI packers accept;
  * it is the QDP++ checkerboarded-lexicographic order that
