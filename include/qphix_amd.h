@@ -396,6 +396,20 @@ int qphix_bicgstab_rxupdate(const qphix_geom *g, void *x, void *r, const void *t
 int qphix_mr_rxupdate(const qphix_geom *g, void *x, void *r, const void *mr,
                       const double a[2]);                                                /* 796 */
 
+/* Reductions into device memory (no reference counterpart: the reference's
+ * BLAS return host doubles; these are for callers that capture solver steps
+ * in HIP graphs). The same operations and values as norm2Spinor (278),
+ * axpyNorm2 (312), xmyNorm2Spinor (345), innerProduct (520) and
+ * innerProductNorm (573) above, with the 1, 1, 1, 2 or 3 results written to
+ * d_out (device memory) by kernels on the library stream: no copy, no memset,
+ * no synchronisation. Calls share the library's partial-sum buffer, so two of
+ * them must not run concurrently on different streams. */
+int qphix_norm2_spinor_dev(const qphix_geom *g, const void *x, double *d_out);
+int qphix_axpy_norm2_dev(const qphix_geom *g, double alpha, const void *x, void *y, double *d_out);
+int qphix_xmy_norm2_dev(const qphix_geom *g, void *res, const void *x, const void *y, double *d_out);
+int qphix_inner_product_dev(const qphix_geom *g, const void *l, const void *r, double *d_out);
+int qphix_inner_product_norm_dev(const qphix_geom *g, const void *l, const void *r, double *d_out);
+
 /* ---- Chroma LinOpSysSolver surface (README.markdown:163-256): the
  *      <InvertParam> group of QPHIX_CLOVER_INVERTER and
  *      QPHIX_CLOVER_ITER_REFINE_BICGSTAB_INVERTER mapped onto this library.

@@ -565,6 +565,20 @@ def xmy_norm2(geom, res, x, y):
     return r.value
 
 
+def set_stream(stream):
+    """Order the library's work on a HIP stream (a hipStream_t as int or
+    c_void_p); None returns to the library's own stream."""
+    call("qphix_set_stream", VP(stream) if isinstance(stream, int) else stream)
+
+
+def reduce_dev(name, geom, d_out, *args):
+    """The device-memory form of a reduction, qphix_<name>_dev: results land in
+    d_out (a device address) without synchronisation."""
+    conv = [a.p if isinstance(a, Field) else (C.c_double(a) if isinstance(a, float) else a)
+            for a in args]
+    call(f"qphix_{name}_dev", geom.h, *conv, VP(d_out) if isinstance(d_out, int) else d_out)
+
+
 def block_gram(geom, a, b):
     """G[i, j] = <a_i, b_j> (complex numpy matrix)."""
     o = np.zeros(len(a) * len(b) * 2)

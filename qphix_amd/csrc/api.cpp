@@ -428,6 +428,14 @@ static int run_blas(const qphix_geom *g, int op, const BlasArgs &a, double *out)
   }
   return QPHIX_OK;
 }
+// the reduction left in device memory: kernels only (no copy, no memset, no
+// synchronisation), so the call can be captured in a HIP graph
+static int run_blas_dev(const qphix_geom *g, int op, const BlasArgs &a, double *d_out) {
+  int nb = 0;
+  HIPCHK(blas_launch(g->d, op, a, g_partials, &nb, g_stream));
+  HIPCHK(blas_finalize(g_partials, nb, blas_nred(op), d_out, g_stream));
+  return QPHIX_OK;
+}
 static BlasArgs bargs() {
   BlasArgs a;
   memset(&a, 0, sizeof(a));
@@ -541,6 +549,33 @@ extern "C" int qphix_inner_product_norm(const qphix_geom *g, const void *l, cons
   BLAS_PRE(); CHKPTR(l); CHKPTR(r); CHKPTR(out);
   BlasArgs a = bargs(); a.x = l; a.y = r;
   return run_blas(g, B_CDOTNORM, a, out);
+}
+
+// reductions into device memory (graph-capturable forms of the above)
+extern "C" int qphix_norm2_spinor_dev(const qphix_geom *g, const void *x, double *d_out) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(d_out);
+  BlasArgs a = bargs(); a.x = x;
+  return run_blas_dev(g, B_NORM2, a, d_out);
+}
+extern "C" int qphix_axpy_norm2_dev(const qphix_geom *g, double alpha, const void *x, void *y, double *d_out) {
+  BLAS_PRE(); CHKPTR(x); CHKPTR(y); CHKPTR(d_out);
+  BlasArgs a = bargs(); a.x = x; a.o0 = y; a.a[0] = alpha;
+  return run_blas_dev(g, B_AXPYNORM2, a, d_out);
+}
+extern "C" int qphix_xmy_norm2_dev(const qphix_geom *g, void *res, const void *x, const void *y, double *d_out) {
+  BLAS_PRE(); CHKPTR(res); CHKPTR(x); CHKPTR(y); CHKPTR(d_out);
+  BlasArgs a = bargs(); a.x = x; a.y = y; a.o0 = res;
+  return run_blas_dev(g, B_XMYNORM2, a, d_out);
+}
+extern "C" int qphix_inner_product_dev(const qphix_geom *g, const void *l, const void *r, double *d_out) {
+  BLAS_PRE(); CHKPTR(l); CHKPTR(r); CHKPTR(d_out);
+  BlasArgs a = bargs(); a.x = l; a.y = r;
+  return run_blas_dev(g, B_CDOT, a, d_out);
+}
+extern "C" int qphix_inner_product_norm_dev(const qphix_geom *g, const void *l, const void *r, double *d_out) {
+  BLAS_PRE(); CHKPTR(l); CHKPTR(r); CHKPTR(d_out);
+  BlasArgs a = bargs(); a.x = l; a.y = r;
+  return run_blas_dev(g, B_CDOTNORM, a, d_out);
 }
 extern "C" int qphix_bicgstab_p_update(const qphix_geom *g, const void *r, void *p, const void *v,
                                        const double beta[2], const double omega[2]) {
