@@ -208,11 +208,15 @@ extern "C" int qphix_geom_create(const int latt[4], int prec, int c12, qphix_geo
   if (prec != QPHIX_F64 && prec != QPHIX_F32 && prec != QPHIX_F16)
     return fail(QPHIX_EDOMAIN, "unknown precision %d", prec);
   if (prec == QPHIX_F16 && !c12) return fail(QPHIX_EDOMAIN, "F16 geometries require two-row gauge compression");
+  // the kernels index sites of one parity (padded to whole 256-site blocks)
+  // and their neighbours in 32 bits
+  const long vh = (long)latt[0] * latt[1] * latt[2] * latt[3] / 2;
+  if (vh > 0x7fffff00L) return fail(QPHIX_EDOMAIN, "a parity of %ld sites exceeds the 32-bit site index", vh);
   qphix_geom *g = new qphix_geom;
   for (int k = 0; k < 4; k++) g->d.L[k] = latt[k];
   g->d.prec = prec;
   g->d.c12 = c12 ? 1 : 0;
-  g->d.vh = (long)latt[0] * latt[1] * latt[2] * latt[3] / 2;
+  g->d.vh = vh;
   g->d.vs = (size_t)((g->d.vh + 255) / 256) * 256;
   const int nr[4] = {kSpinorReals, gauge_reals(c12 != 0), kCloverReals, kFullCloverReals};
   for (int k = 0; k < 4; k++) {
@@ -489,6 +493,7 @@ extern "C" int qphix_twisted_mass(const qphix_geom *g, const double apimu[2], co
 extern "C" int qphix_two_flav_twisted_mass(const qphix_geom *g, const double apimu[2], double epsilon,
                                            const void *const x[2], void *const y[2]) {
   BLAS_PRE(); CHKPTR(apimu); CHKPTR(x); CHKPTR(y);
+  CHKPTR(x[0]); CHKPTR(x[1]); CHKPTR(y[0]); CHKPTR(y[1]);
   BlasArgs a = bargs(); a.x = x[0]; a.y = x[1]; a.o0 = y[0]; a.o1 = y[1];
   a.a[0] = apimu[0]; a.a[1] = apimu[1]; a.a[2] = epsilon;
   return run_blas(g, B_TWM2, a, nullptr);
@@ -1059,6 +1064,7 @@ extern "C" int qphix_tmclov_two_flav_inverse_clover_term(qphix_dslash *d, void *
                                                          const void *cl_offdiag, int isign) {
   ENSURE_INIT();
   CHKPTR(d); CHKPTR(res); CHKPTR(psi); CHKPTR(fcl); CHKPTR(cl_offdiag);
+  CHKPTR(res[0]); CHKPTR(res[1]); CHKPTR(psi[0]); CHKPTR(psi[1]); CHKPTR(fcl[0]); CHKPTR(fcl[1]);
   BlasArgs a = bargs();
   a.x = psi[0];
   a.y = psi[1];
@@ -1215,7 +1221,10 @@ extern "C" int qphix_norm2_multi(const qphix_geom *g, const void *const x[], int
   CHKPTR(g); CHKPTR(x); CHKPTR(n2);
   if (n < 1 || n > kMaxRhs) return fail(QPHIX_EDOMAIN, "n = %d right-hand sides (max %d)", n, kMaxRhs);
   MultiBlasArgs m = margs(n);
-  for (int r = 0; r < n; r++) m.x[r] = x[r];
+  for (int r = 0; r < n; r++) {
+    if (!x[r]) return fail(QPHIX_EINVAL, "null field %d", r);
+    m.x[r] = x[r];
+  }
   return run_blas_multi(g, B_NORM2, m, n2);
 }
 

@@ -26,6 +26,9 @@ def test_geometry_validation_without_gpu():
     assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), q.F16, 0, C.byref(h)) == q.EDOMAIN
     assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), 7, 0, C.byref(h)) == q.EDOMAIN
     assert lib.qphix_geom_create(None, q.F64, 0, C.byref(h)) == q.EINVAL
+    # parities beyond the kernels' 32-bit site index
+    assert lib.qphix_geom_create((C.c_int * 4)(256, 256, 256, 256), q.F32, 1, C.byref(h)) == q.EDOMAIN
+    assert b"32-bit" in lib.qphix_last_error()
 
 
 @pytest.mark.parametrize("prec,c12,expect", [
