@@ -59,11 +59,11 @@ QPHIX_T void richardson_rxupdateNormR(QPHIX_SP *x, QPHIX_SP *r, const QPHIX_SP *
   QPHIX_C(qphix_richardson_rxupdate_norm_r(geom.handle(), x, r, delta_x, delta_r, &r_norm));
 }
 QPHIX_T void bicgstab_xmy(const QPHIX_SP *x, QPHIX_SP *y, QPHIX_G, int = 1) { QPHIX_C(qphix_bicgstab_xmy(geom.handle(), x, y)); }
-QPHIX_T void innerProduct(double results[2], const QPHIX_SP *l, const QPHIX_SP *r, QPHIX_G, int = 1)
+QPHIX_T void innerProduct(double results[2], const QPHIX_SP *l, const QPHIX_SP *r, QPHIX_G, int = 1, bool = true)
 {
   QPHIX_C(qphix_inner_product(geom.handle(), l, r, results));
 }
-QPHIX_T void innerProductNorm(double results[3], const QPHIX_SP *l, const QPHIX_SP *r, QPHIX_G, int = 1)
+QPHIX_T void innerProductNorm(double results[3], const QPHIX_SP *l, const QPHIX_SP *r, QPHIX_G, int = 1, bool = true)
 {
   QPHIX_C(qphix_inner_product_norm(geom.handle(), l, r, results));
 }

@@ -124,4 +124,48 @@ void pack_full_clover(const Geometry<FT, V, S, C> &g, typename Geometry<FT, V, S
   detail::check(qphix_pack_full_clover(g.handle(), dst, src, cb), "pack_full_clover");
 }
 
+// Owning handles of one checkerboard field (geometry.h:443-548): allocated
+// from the geometry on construction, freed on destruction, movable.
+namespace detail {
+template <class G, class Block, Block *(G::*Alloc)()> class FieldHandle {
+ public:
+  typedef G Geom;
+  typedef Block ValueType;
+  explicit FieldHandle(Geom &geom) : value_((geom.*Alloc)()), geom_(geom) {}
+  FieldHandle(FieldHandle &&other) noexcept : value_(other.value_), geom_(other.geom_) { other.value_ = nullptr; }
+  FieldHandle(const FieldHandle &) = delete;
+  FieldHandle &operator=(const FieldHandle &) = delete;
+  ~FieldHandle()
+  {
+    if (value_) geom_.free(value_);
+  }
+  ValueType *get() const { return value_; }
+
+ private:
+  ValueType *value_;
+  Geom &geom_;
+};
+}  // namespace detail
+
+#define QPHIX_HANDLE(Name, Block, Alloc)                                                                          \
+  template <typename FT, int V, int S, bool C>                                                                    \
+  class Name : public detail::FieldHandle<Geometry<FT, V, S, C>, typename Geometry<FT, V, S, C>::Block,           \
+                                          &Geometry<FT, V, S, C>::Alloc> {                                       \
+    typedef detail::FieldHandle<Geometry<FT, V, S, C>, typename Geometry<FT, V, S, C>::Block,                     \
+                                &Geometry<FT, V, S, C>::Alloc>                                                    \
+        Base;                                                                                                     \
+                                                                                                                  \
+   public:                                                                                                        \
+    using Base::Base;                                                                                             \
+  };                                                                                                              \
+  template <typename FT, int V, int S, bool C> Name<FT, V, S, C> make##Name(Geometry<FT, V, S, C> &geom)          \
+  {                                                                                                               \
+    return Name<FT, V, S, C>(geom);                                                                               \
+  }
+QPHIX_HANDLE(FourSpinorHandle, FourSpinorBlock, allocCBFourSpinor)
+QPHIX_HANDLE(GaugeHandle, SU3MatrixBlock, allocCBGauge)
+QPHIX_HANDLE(CloverHandle, CloverBlock, allocCBClov)
+QPHIX_HANDLE(FullCloverHandle, FullCloverBlock, allocCBFullClov)
+#undef QPHIX_HANDLE
+
 }  // namespace QPhiX

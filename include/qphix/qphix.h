@@ -8,3 +8,7 @@
 #include "unprec_solver_wrapper.h"
 #include "syssolver_linop_clover.h"
 #include "blas_new_c.h"
+#include "blas_full_spinor.h"
+#include "arith_type.h"
+#include "clover_product.h"
+#include "inv_dummy_hermtest.h"

@@ -170,6 +170,10 @@ int qphix_clov_dslash(qphix_dslash *d, void *res, const void *psi, const void *u
 int qphix_clov_achimbdpsi(qphix_dslash *d, void *res, const void *psi, const void *chi,
                           const void *u, const void *clov, double beta, int isign, int cb);
 int qphix_clov_mult(qphix_dslash *d, void *res, const void *chi, const void *clov);
+/* res = A chi for the packed Hermitian clover blocks (full = 0) or the full
+ * complex 6x6 blocks (full = 1) of one checkerboard: the site product of
+ * clover_product.h:251-339 (there accumulated into res; here assigned) */
+int qphix_clover_apply(const qphix_geom *g, void *res, const void *chi, const void *clov, int full);
 /* TMDslash::dslash, ::dslashAChiMinusBDPsi, ::two_flav_AChiMinusBDPsi
  * (tm_dslash_def.h:35-58) */
 int qphix_tm_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign,

@@ -1014,6 +1014,15 @@ extern "C" int qphix_clov_mult(qphix_dslash *d, void *res, const void *chi, cons
   BlasArgs a = bargs(); a.x = chi; a.y = clov; a.o0 = res;
   return run_blas(d->g, B_CLOVMULT, a, nullptr);
 }
+// res = A chi with the packed Hermitian blocks (full = 0) or the full complex
+// blocks (full = 1) of one checkerboard
+extern "C" int qphix_clover_apply(const qphix_geom *g, void *res, const void *chi, const void *clov, int full) {
+  ENSURE_INIT();
+  CHKPTR(g); CHKPTR(res); CHKPTR(chi); CHKPTR(clov);
+  if (full != 0 && full != 1) return fail(QPHIX_EDOMAIN, "full must be 0 or 1, got %d", full);
+  BlasArgs a = bargs(); a.x = chi; a.y = clov; a.o0 = res;
+  return run_blas(g, full ? B_FULLMULT : B_CLOVMULT, a, nullptr);
+}
 extern "C" int qphix_tm_dslash(qphix_dslash *d, void *res, const void *psi, const void *u, int isign, int cb) {
   CHKPTR(d);
   const double al = 4.0 + d->p.mass, mu = d->p.twisted_mass;

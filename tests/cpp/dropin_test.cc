@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "qphix/qphix.h"
@@ -147,6 +148,71 @@ int main()
       geom.free(xs[s][0]);
     }
     check("minvcg_worst_shift_true_resid", worst, 2e-10);
+  }
+  // field handles, ArithType, full-spinor BLAS, clover_product, InvDummyHermTest
+  {
+    static_assert(std::is_same<ArithType<half>::Type, float>::value, "half computes in float");
+    static_assert(std::is_same<ArithType<double>::Type, double>::value, "double computes in double");
+    FourSpinorHandle<double, 8, 4, false> h1(geom);
+    auto h2 = makeFourSpinorHandle(geom);
+    copySpinor(h1.get(), b, geom, 1);
+    double n1;
+    norm2Spinor(n1, h1.get(), geom, 1);
+    check("handle_copy_norm", std::fabs(n1 - bnorm) / bnorm, 0.0);
+    // full spinors: the norm is the sum over checkerboards, ax scales both
+    FullSpinor<double, 8, 4, false> fs(geom), ft(geom);
+    detail::check(qphix_gen_spinor(geom.handle(), fs.getCBData(0), 31, 0), "gen_spinor");
+    copySpinor(fs.getCBData(1), b, geom, 1);
+    double n0, nf;
+    norm2Spinor(n0, fs.getCBData(0), geom, 1);
+    norm2Spinor(nf, fs, geom, 1);
+    check("full_norm2_sums_cbs", std::fabs(nf - n0 - bnorm) / nf, 1e-15);
+    copySpinor(ft, fs, geom, 1);
+    axSpinor(2.0, ft, geom, 1);
+    double ip[2];
+    innerProductSpinor(ip, fs, ft, geom, 1);
+    check("full_inner_product", std::fabs(ip[0] - 2.0 * nf) / nf + std::fabs(ip[1]) / nf, 1e-14);
+    double xr;
+    xmy2Norm2Spinor(fs, ft, xr, geom, 1);  // ft = 2 fs - fs = fs
+    check("full_xmy2norm2", std::fabs(xr - nf) / nf, 1e-14);
+    // clover_product accumulates A in into out
+    GaugeHandle<double, 8, 4, false> gh(geom);
+    CloverHandle<double, 8, 4, false> ch(geom);
+    FullCloverHandle<double, 8, 4, false> fh(geom);
+    ClovDslash<double, 8, 4, false> D(&geom, t_bc, 1.0, 1.0);
+    copySpinor(x, b, geom, 1);
+    clover_product<double, 8, 4, false>(x, b, clov, geom);  // x = b + A b
+    D.clovMult(t, b, clov);
+    ypeqx(b, t, geom, 1);
+    double n;
+    xmyNorm2Spinor(r, x, t, n, geom, 1);
+    check("clover_product_accumulates", std::sqrt(n / bnorm), 1e-14);
+    detail::check(qphix_clover_build(geom.handle(), u[0], u[1], csw, csw, 4 + mass, 1, QPHIX_CLOV_TMINV, 0.05, 0,
+                                     fh.get()),
+                  "full clover");
+    zeroSpinor(x, geom, 1);
+    clover_product<double, 8, 4, false>(x, b, fh.get(), geom);  // (A + i mu g5)^-1 b
+    // applying A + i mu g5 gives b back: A x + i mu g5 x
+    D.clovMult(t, x, clov);
+    const double pimu[2] = {0.0, 0.05};
+    copySpinor(r, x, geom, 1);
+    twisted_mass(pimu, x, r, geom, 1);  // r = i mu g5 x
+    ypeqx(r, t, geom, 1);
+    xmyNorm2Spinor(r, t, b, n, geom, 1);
+    check("clover_product_full_block_inverse", std::sqrt(n / bnorm), 1e-13);
+    (void)gh; (void)ch; (void)h2;
+    // the hermiticity test sees <x|M^+M y> = <Mx|My>
+    InvDummyHermTest<double, 8, 4, false> herm(M, 1);
+    detail::check(qphix_gen_spinor(geom.handle(), x, 41, 1), "gen_spinor");
+    int it;
+    double rsd = -1;
+    unsigned long fl, mv;
+    herm(x, b, 1e-10, it, rsd, fl, mv, 1, false);
+    double a1[2], a2[2];
+    herm.lastProducts(a1, a2);
+    const double sc = std::fabs(a2[0]) + std::fabs(a2[1]);
+    check("dummy_hermtest_products_agree", (std::fabs(a1[0] - a2[0]) + std::fabs(a1[1] - a2[1])) / sc, 1e-12);
+    check("dummy_hermtest_counts", (it == 1 && mv == 3) ? 0.0 : 1.0, 0.0);
   }
   // errors map to the reference's exceptions
   {

@@ -254,6 +254,30 @@ def test_gpu_anisotropic_clover_build_matches_oracle():
         assert rel(ai.download(), oracle.clover_invert(ch)) < 1e-12
 
 
+@pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True)], ids=["f64", "f32", "f16"])
+def test_clover_apply_entry_point(prec, c12):
+    """qphix_clover_apply: A x with packed Hermitian blocks and F x with full
+    complex blocks (the site product of clover_product.h)."""
+    L = [4, 4, 4, 8]
+    vh = 256
+    g = q.Geometry(L, prec, c12)
+    u = oracle.gen_gauge(L, 11, 0.4)
+    clov = oracle.clover_build(L, u, 1.0, 0.1)[:vh * 72]
+    fcl = oracle.full_clover_tm_inverse(clov, 0.05)
+    x = g.spinor().upload(oracle.gen_spinor(L, 12, 0), 0)
+    xh = x.download(0)
+    cd = g.clover().upload(clov)
+    fd = g.full_clover().upload(fcl)
+    out = g.spinor()
+    tol = TOL[prec] * (4 if prec == q.F16 else 1)
+    q.call("qphix_clover_apply", g.h, out.p, x.p, cd.p, 0)
+    assert rel(out.download(0), oracle.clover_apply(clov, xh)) < tol
+    q.call("qphix_clover_apply", g.h, out.p, x.p, fd.p, 1)
+    assert rel(out.download(0), oracle.full_clover_apply(fcl, xh)) < tol
+    with pytest.raises(q.QphixError):
+        q.call("qphix_clover_apply", g.h, out.p, x.p, fd.p, 2)
+
+
 def test_precision_conversion():
     L = [4, 4, 4, 8]
     g64 = q.Geometry(L, q.F64, False)
