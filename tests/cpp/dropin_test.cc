@@ -213,6 +213,8 @@ int main()
     const double sc = std::fabs(a2[0]) + std::fabs(a2[1]);
     check("dummy_hermtest_products_agree", (std::fabs(a1[0] - a2[0]) + std::fabs(a1[1] - a2[1])) / sc, 1e-12);
     check("dummy_hermtest_counts", (it == 1 && mv == 3) ? 0.0 : 1.0, 0.0);
+    masterPrintf("peak host memory %.1f MiB\n", get_max_resident_mib());
+    check("resident_memory_reported", get_max_resident_mib() > 1.0 ? 0.0 : 1.0, 0.0);
   }
   // errors map to the reference's exceptions
   {
