@@ -11,6 +11,7 @@ one 72-real clover block: (8*12 + 24 + 24 + 72) * 4 B = 864 B (FP32).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
 """
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -158,6 +159,19 @@ def main():
     gbs18 = BYTES_SITE[("f32", False)] * vh / (ms18 * 1e-3) / 1e9
     log(f"clover dslash f32 c18: {ms18:.4f} ms/launch, {gbs18:.1f} GB/s algorithmic")
 
+    # the same per-site traffic without the stencil (one work-item per site
+    # streaming 48 words in and 6 out, 60 / 6 for c18): the bandwidth this
+    # traffic shape reaches on the device, the practical ceiling of the hop
+    ceiling = {}
+    for name, wi, wo, nb, hop_ms in (("f32_c12", 48, 6, BYTES_SITE[("f32", True)], per_ms),
+                                     ("f32_c18", 60, 6, BYTES_SITE[("f32", False)], ms18)):
+        v = ctypes.c_double()
+        q.call("qphix_stream_calibrate", g32.h, wi, wo, 20, ctypes.byref(v))
+        cg = nb * vh / (v.value * 1e-3) / 1e9
+        ceiling[name] = {"ms": round(v.value, 5), "gbs": round(cg, 1), "frac_of_peak": round(cg / HBM_PEAK_GBS, 4),
+                         "hop_over_ceiling": round(v.value / hop_ms, 4)}
+        log(f"streaming ceiling {name}: {v.value:.4f} ms, {cg:.1f} GB/s; hop at {v.value / hop_ms:.1%} of it")
+
     # multi-right-hand-side clover Dslash: 12 right-hand sides share every link
     # and clover read; algorithmic bytes 8*G + C + n*(S_in + S_out) per site
     nr = 12
@@ -172,6 +186,7 @@ def main():
         q.copy(g32, out, psi)
     q.sync()
 
+    extra["streaming_ceiling"] = ceiling
     extra.update({"tuned": q.tune_summary().strip().splitlines(),
                   "ms_per_dslash_c12": round(per_ms, 5), "ms_per_dslash_c18": round(ms18, 5),
                   "gbs_c18": round(gbs18, 1),

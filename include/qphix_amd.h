@@ -414,6 +414,15 @@ int qphix_xmy_norm2_dev(const qphix_geom *g, void *res, const void *x, const voi
 int qphix_inner_product_dev(const qphix_geom *g, const void *l, const void *r, double *d_out);
 int qphix_inner_product_norm_dev(const qphix_geom *g, const void *l, const void *r, double *d_out);
 
+/* Streaming calibration (no reference counterpart; measurement only): mean
+ * time of a kernel that moves a hop kernel's per-site traffic without its
+ * stencil, one work-item per site of g reading words_in 16-byte words and
+ * writing words_out in the site-blocked field layout with the hop kernels'
+ * load / store policies. (48, 6) is the FP32 c12 clover hop (links 24, clover
+ * 18, spinor 6 in; 6 out), (60, 6) FP32 c18, (120, 12) FP64 c18. Allocates
+ * and frees its own buffers; synchronous. */
+int qphix_stream_calibrate(const qphix_geom *g, int words_in, int words_out, int reps, double *ms);
+
 /* ---- Chroma LinOpSysSolver surface (README.markdown:163-256): the
  *      <InvertParam> group of QPHIX_CLOVER_INVERTER and
  *      QPHIX_CLOVER_ITER_REFINE_BICGSTAB_INVERTER mapped onto this library.

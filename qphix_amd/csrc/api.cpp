@@ -432,6 +432,34 @@ static int run_blas(const qphix_geom *g, int op, const BlasArgs &a, double *out)
   }
   return QPHIX_OK;
 }
+extern "C" int qphix_stream_calibrate(const qphix_geom *g, int words_in, int words_out, int reps, double *ms) {
+  CHKPTR(g); CHKPTR(ms);
+  const bool ok = (words_in == 48 && words_out == 6) || (words_in == 60 && words_out == 6) ||
+                  (words_in == 120 && words_out == 12);
+  if (!ok) return fail(QPHIX_EDOMAIN, "stream shape %d / %d words per site not built (48/6, 60/6, 120/12)",
+                       words_in, words_out);
+  if (reps < 1) return fail(QPHIX_EDOMAIN, "reps %d", reps);
+  ENSURE_INIT();
+  void *in = nullptr, *out = nullptr;
+  TRY(alloc_zeroed((size_t)words_in * g->d.vs * 16, &in));
+  int rc = alloc_zeroed((size_t)words_out * g->d.vs * 16, &out);
+  float f = 0;
+  if (!rc) {
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = stream_launch(g->d, words_in, words_out, in, out, g_stream);
+    if (e == hipSuccess) e = hipEventRecord(g_ev0, g_stream);
+    for (int k = 0; k < reps && e == hipSuccess; k++) e = stream_launch(g->d, words_in, words_out, in, out, g_stream);
+    if (e == hipSuccess) e = hipEventRecord(g_ev1, g_stream);
+    if (e == hipSuccess) e = hipEventSynchronize(g_ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, g_ev0, g_ev1);
+    if (e != hipSuccess) rc = fail(QPHIX_EHIP, "stream calibration: %s", hipGetErrorString(e));
+  }
+  qphix_free(in);
+  qphix_free(out);
+  if (!rc) *ms = f / reps;
+  return rc;
+}
+
 // the reduction left in device memory: kernels only (no copy, no memset, no
 // synchronisation), so the call can be captured in a HIP graph
 static int run_blas_dev(const qphix_geom *g, int op, const BlasArgs &a, double *d_out) {

@@ -486,3 +486,36 @@ hipError_t blas_finalize(const double *partials, int nblocks, int nred, double *
 }
 
 }  // namespace qphix_amd
+
+// ---------------------------------------------------------------------------
+// Streaming calibration: one work-item per site reads NWI 16-byte words of one
+// field and writes NWO words of another, both in the site-blocked layout of
+// every field, with the hop kernels' load / store policies. It moves a hop
+// kernel's per-site traffic without the stencil, which measures the bandwidth
+// that traffic shape can reach on the device.
+// ---------------------------------------------------------------------------
+namespace qphix_amd {
+
+template <int NWI, int NWO>
+__global__ void __launch_bounds__(256) k_stream(const f4_t *in, f4_t *out, long n) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  f4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int w = 0; w < NWI; w++) acc += __builtin_nontemporal_load(in + widx(s, w, NWI));
+#pragma unroll
+  for (int w = 0; w < NWO; w++) __builtin_nontemporal_store(acc * (float)(w + 1), out + widx(s, w, NWO));
+}
+
+hipError_t stream_launch(const GeomDesc &g, int nwi, int nwo, const void *in, void *out, hipStream_t st) {
+  const unsigned grid = (unsigned)((g.vh + 255) / 256);
+  const f4_t *i = (const f4_t *)in;
+  f4_t *o = (f4_t *)out;
+  if (nwi == 48 && nwo == 6) k_stream<48, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
+  else if (nwi == 60 && nwo == 6) k_stream<60, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
+  else if (nwi == 120 && nwo == 12) k_stream<120, 12><<<grid, 256, 0, st>>>(i, o, g.vh);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace qphix_amd

@@ -71,3 +71,13 @@ def test_tune_rhs_validation():
     assert lib.qphix_tune_rhs(2, 1) == q.EDOMAIN
     assert lib.qphix_tune_rhs(0, 0) == q.OK
     assert lib.qphix_tune_config(2, 100, 0, 1) == q.EDOMAIN  # block not a multiple of 64
+
+
+def test_stream_calibrate_validation():
+    """Only the built traffic shapes are accepted (host-side check, no GPU)."""
+    lib = q.lib()
+    g = q.Geometry([8, 8, 8, 8], q.F32, 1)
+    d = C.c_double()
+    assert lib.qphix_stream_calibrate(g.h, 47, 6, 5, C.byref(d)) == q.EDOMAIN
+    assert lib.qphix_stream_calibrate(g.h, 48, 6, 0, C.byref(d)) == q.EDOMAIN
+    assert lib.qphix_stream_calibrate(g.h, 48, 6, 5, None) == q.EINVAL
