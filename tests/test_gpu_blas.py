@@ -160,3 +160,13 @@ def test_reduction_deterministic_and_large():
     assert vals[0] == vals[1] == vals[2]
     xh = x.download()
     assert abs(vals[0] - np.sum(xh * xh)) / vals[0] < 1e-13
+
+
+def test_stream_calibrate_runs():
+    """The streaming calibration kernel runs for every built traffic shape and
+    reports a positive mean launch time."""
+    g = q.Geometry([8, 8, 8, 16], q.F32, True)
+    for wi, wo in ((48, 6), (60, 6), (120, 12)):
+        ms = C.c_double()
+        q.call("qphix_stream_calibrate", g.h, wi, wo, 3, C.byref(ms))
+        assert 0 < ms.value < 100.0, (wi, wo, ms.value)

@@ -185,3 +185,17 @@ def test_initial_guess_is_used(gauge, chi):
     r1 = S(psi, chi)
     r2 = S(psi, chi)  # starts from the solution
     assert r2.status == q.OK and r2.n_count < r1.n_count
+
+
+def test_single_precision_outer(gauge, chi):
+    """<OuterPrecision>f</OuterPrecision>: the whole solve in FP32 storage (the
+    reference built with single-precision QDP++ types); the residual check
+    then runs in FP32 storage with FP64 sums, so targets stay above 1e-7."""
+    x = xml(rsd=1e-6, extra="<OuterPrecision>f</OuterPrecision><Compress12>true</Compress12>"
+                            "<AntiPeriodicT>true</AntiPeriodicT>")
+    S = q.SysSolver(L, x, gauge)
+    psi = np.zeros(2 * VH * 24)
+    r = S(psi, chi)
+    assert r.status == q.OK
+    Mo = oracle_op(gauge, t_boundary=-1.0, **ISO)
+    assert rel(Mo(psi[VH * 24:], 1, 1), chi[VH * 24:]) < 5e-6
