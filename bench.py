@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-block", action="store_true")
+    ap.add_argument("--no-family", action="store_true",
+                    help="skip the Wilson and twisted-mass clover Dslash measurements")
     ap.add_argument("--no-multi", action="store_true",
                     help="skip the 12-RHS launches (they share the headline kernel's symbol)")
     ap.add_argument("--rhs-tile", default=None,
@@ -179,6 +181,10 @@ def main():
     if not args.no_multi:
         extra["multi_rhs_12"] = multi_rhs(q, D, g32, u32, inv32, nr, max(1, args.steps // 10),
                                           per_ms, timer)
+
+    if not args.no_family:
+        extra["kernels_48x48x48x96"] = kernel_family(q, g64, u64, g32, u32, psi, out, vh,
+                                                     max(10, args.steps // 5))
 
     # copy kernel of known size (FP32 spinor, 96 B/site read + 96 B/site written)
     # for the profiler's FETCH_SIZE / WRITE_SIZE calibration; not timed
@@ -274,6 +280,38 @@ def multi_rhs(q, D, g32, u32, inv32, nr, reps, per_ms, timer):
     log(f"clover dslash f32 c12, {nm} rhs on matrix cores: {ms_m:.3f} ms ({ms_m / nm:.4f} ms per rhs), "
         f"MFMA {issued / (ms_m * 1e-3) / 1e12:.2f} TFLOP/s issued")
     return out
+
+
+def kernel_family(q, g64, u64, g32, u32, psi, out, vh, reps):
+    """The FP32 two-row Wilson Dslash (576 B/site) and twisted-mass clover
+    Dslash with full inverse blocks (1152 B/site, mu = 0.01) on the same
+    48^3x96 fields, each beside the streaming calibration of its traffic."""
+    mu = 0.01
+    full64 = q.clover_build(g64, u64, CSW, MASS, 0, q.CLOV_TMINV, mu)
+    full32 = q.convert_clover(g32, g64, full64, full=True)
+    del full64
+    D = q.Dslash(g32, t_boundary=-1.0, prec_mass_rho=mu)
+    timer = q.Timer()
+    res = {}
+    cases = (("wilson_f32c12", 576, 30, lambda: D.dslash(out, psi, u32[0], 1, 0)),
+             ("tmclover_f32c12_full_blocks", 1152, 66,
+              lambda: D.tmclov_dslash(out, psi, u32[0], [full32, full32], 1, 0, 0)))
+    for name, nb, words, launch in cases:
+        for _ in range(5):
+            launch()
+        timer.start()
+        for _ in range(reps):
+            launch()
+        ms = timer.stop() / reps
+        c = ctypes.c_double()
+        q.call("qphix_stream_calibrate", g32.h, words, 6, 20, ctypes.byref(c))
+        gb = nb * vh / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 5), "bytes_per_site": nb, "gbs_algorithmic": round(gb, 1),
+                     "frac_of_peak": round(gb / HBM_PEAK_GBS, 4), "streaming_ceiling_ms": round(c.value, 5),
+                     "hop_over_ceiling": round(c.value / ms, 4)}
+        log(f"{name}: {ms:.4f} ms, {gb:.1f} GB/s ({gb / HBM_PEAK_GBS:.1%} of peak), "
+            f"{c.value / ms:.1%} of its streaming ceiling")
+    return res
 
 
 def doublet_operator(q, reps=50):

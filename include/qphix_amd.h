@@ -419,8 +419,10 @@ int qphix_inner_product_norm_dev(const qphix_geom *g, const void *l, const void 
  * stencil, one work-item per site of g reading words_in 16-byte words and
  * writing words_out in the site-blocked field layout with the hop kernels'
  * load / store policies. (48, 6) is the FP32 c12 clover hop (links 24, clover
- * 18, spinor 6 in; 6 out), (60, 6) FP32 c18, (120, 12) FP64 c18. Allocates
- * and frees its own buffers; synchronous. */
+ * 18, spinor 6 in; 6 out), (60, 6) FP32 c18, (30, 6) the FP32 c12 Wilson hop,
+ * (66, 6) the FP32 c12 twisted-mass clover hop (full blocks 36 words),
+ * (120, 12) the FP64 c18 clover hop. Allocates and frees its own buffers;
+ * synchronous. */
 int qphix_stream_calibrate(const qphix_geom *g, int words_in, int words_out, int reps, double *ms);
 
 /* ---- Chroma LinOpSysSolver surface (README.markdown:163-256): the

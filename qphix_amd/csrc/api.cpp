@@ -434,10 +434,11 @@ static int run_blas(const qphix_geom *g, int op, const BlasArgs &a, double *out)
 }
 extern "C" int qphix_stream_calibrate(const qphix_geom *g, int words_in, int words_out, int reps, double *ms) {
   CHKPTR(g); CHKPTR(ms);
-  const bool ok = (words_in == 48 && words_out == 6) || (words_in == 60 && words_out == 6) ||
+  const bool ok = (words_out == 6 && (words_in == 30 || words_in == 48 || words_in == 60 || words_in == 66)) ||
                   (words_in == 120 && words_out == 12);
-  if (!ok) return fail(QPHIX_EDOMAIN, "stream shape %d / %d words per site not built (48/6, 60/6, 120/12)",
-                       words_in, words_out);
+  if (!ok)
+    return fail(QPHIX_EDOMAIN, "stream shape %d / %d words per site not built (30, 48, 60, 66 / 6; 120 / 12)",
+                words_in, words_out);
   if (reps < 1) return fail(QPHIX_EDOMAIN, "reps %d", reps);
   ENSURE_INIT();
   void *in = nullptr, *out = nullptr;

@@ -511,7 +511,9 @@ hipError_t stream_launch(const GeomDesc &g, int nwi, int nwo, const void *in, vo
   const unsigned grid = (unsigned)((g.vh + 255) / 256);
   const f4_t *i = (const f4_t *)in;
   f4_t *o = (f4_t *)out;
-  if (nwi == 48 && nwo == 6) k_stream<48, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
+  if (nwi == 30 && nwo == 6) k_stream<30, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
+  else if (nwi == 48 && nwo == 6) k_stream<48, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
+  else if (nwi == 66 && nwo == 6) k_stream<66, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
   else if (nwi == 60 && nwo == 6) k_stream<60, 6><<<grid, 256, 0, st>>>(i, o, g.vh);
   else if (nwi == 120 && nwo == 12) k_stream<120, 12><<<grid, 256, 0, st>>>(i, o, g.vh);
   else return hipErrorInvalidValue;

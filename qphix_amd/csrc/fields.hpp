@@ -84,7 +84,7 @@ struct MultiBlasArgs {
 hipError_t blas_multi_launch(const GeomDesc &g, int op, const MultiBlasArgs &m, double *partials, double *out,
                              hipStream_t st);
 
-// streaming calibration kernel (words per site in / out: 48/6, 60/6, 120/12)
+// streaming calibration kernel (words per site in / out: 30, 48, 60, 66 / 6; 120 / 12)
 hipError_t stream_launch(const GeomDesc &g, int nwi, int nwo, const void *in, void *out, hipStream_t st);
 
 // records msg as the thread's last error (qphix_last_error) and returns code
