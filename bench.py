@@ -90,7 +90,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-block", action="store_true")
     ap.add_argument("--no-family", action="store_true",
-                    help="skip the Wilson and twisted-mass clover Dslash measurements")
+                    help="skip the Wilson and twisted-mass clover Dslash and the local-volume "
+                         "(t-split) measurements")
     ap.add_argument("--no-multi", action="store_true",
                     help="skip the 12-RHS launches (they share the headline kernel's symbol)")
     ap.add_argument("--rhs-tile", default=None,
@@ -185,6 +186,7 @@ def main():
     if not args.no_family:
         extra["kernels_48x48x48x96"] = kernel_family(q, g64, u64, g32, u32, psi, out, vh,
                                                      max(10, args.steps // 5))
+        extra["local_volumes_t_split"] = local_volumes(q, max(20, args.steps // 2))
 
     # copy kernel of known size (FP32 spinor, 96 B/site read + 96 B/site written)
     # for the profiler's FETCH_SIZE / WRITE_SIZE calibration; not timed
@@ -311,6 +313,37 @@ def kernel_family(q, g64, u64, g32, u32, psi, out, vh, reps):
                      "hop_over_ceiling": round(c.value / ms, 4)}
         log(f"{name}: {ms:.4f} ms, {gb:.1f} GB/s ({gb / HBM_PEAK_GBS:.1%} of peak), "
             f"{c.value / ms:.1%} of its streaming ceiling")
+    return res
+
+
+def local_volumes(q, reps):
+    """The FP32 two-row clover Dslash on one GPU's share of 48^3x96 split 2, 4
+    and 8 ways in t (48^3x48, 48^3x24, 48^3x12), periodic wrap standing in for
+    the halo exchange: the per-GPU kernel of a time-sharded run."""
+    res = {}
+    timer = q.Timer()
+    for nt in (48, 24, 12):
+        L = [48, 48, 48, nt]
+        g64 = q.Geometry(L, q.F64, False)
+        u64 = q.gen_gauge(g64, 3, 0.4)
+        inv64 = q.clover_build(g64, u64, CSW, MASS, 0, q.CLOV_AINV)
+        g = q.Geometry(L, q.F32, True)
+        u = q.convert_gauge(g, g64, u64)
+        inv = q.convert_clover(g, g64, inv64)
+        del u64, inv64
+        psi = q.gen_spinor(g, 5, 1)
+        out = g.spinor()
+        D = q.Dslash(g, t_boundary=1.0)
+        for _ in range(5):
+            D.clov_dslash(out, psi, u[0], inv, 1, 0)
+        timer.start()
+        for _ in range(reps):
+            D.clov_dslash(out, psi, u[0], inv, 1, 0)
+        ms = timer.stop() / reps
+        gb = BYTES_SITE[("f32", True)] * g.vh / (ms * 1e-3) / 1e9
+        res[f"48x48x48x{nt}"] = {"split": 96 // nt, "sites_per_parity": g.vh, "ms": round(ms, 5),
+                                 "gbs_algorithmic": round(gb, 1), "frac_of_peak": round(gb / HBM_PEAK_GBS, 4)}
+        log(f"local volume 48^3x{nt} ({96 // nt}-way t split): {ms:.4f} ms, {gb:.1f} GB/s ({gb / HBM_PEAK_GBS:.1%})")
     return res
 
 
