@@ -445,16 +445,20 @@ extern "C" int qphix_stream_calibrate(const qphix_geom *g, int words_in, int wor
   TRY(alloc_zeroed((size_t)words_in * g->d.vs * 16, &in));
   int rc = alloc_zeroed((size_t)words_out * g->d.vs * 16, &out);
   float f = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;  // its own events: a caller's qphix_timer_* may be running
   if (!rc) {
-    hipError_t e = hipSuccess;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = stream_launch(g->d, words_in, words_out, in, out, g_stream);
-    if (e == hipSuccess) e = hipEventRecord(g_ev0, g_stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, g_stream);
     for (int k = 0; k < reps && e == hipSuccess; k++) e = stream_launch(g->d, words_in, words_out, in, out, g_stream);
-    if (e == hipSuccess) e = hipEventRecord(g_ev1, g_stream);
-    if (e == hipSuccess) e = hipEventSynchronize(g_ev1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&f, g_ev0, g_ev1);
+    if (e == hipSuccess) e = hipEventRecord(e1, g_stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
     if (e != hipSuccess) rc = fail(QPHIX_EHIP, "stream calibration: %s", hipGetErrorString(e));
   }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   qphix_free(in);
   qphix_free(out);
   if (!rc) *ms = f / reps;
