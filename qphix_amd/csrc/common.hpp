@@ -131,13 +131,27 @@ template <int P> __device__ inline float site_scale(const void *base, size_t vs,
   return sc[s];
 }
 
+// An empty asm with a read-write register operand: the value must exist in a
+// register as computed, so the instruction that produced it cannot be folded
+// into its users. Emits no instruction.
+__device__ inline void keep_reg(float &x) { asm("" : "+v"(x)); }
+
 // Load reals [W0*per_word, (W0+NW)*per_word) of a field with NWT words per site.
-template <int P, int NWT, int W0, int NW, bool NT = false>
+// SCALED = false leaves FP16 values unscaled (the caller applies site_scale).
+// Unscaled half-to-float conversions stay instructions of their own: folded
+// into mixed-precision FMAs, which the compiler does or not depending on the
+// kernel's shape, they changed the last bits of some results between the
+// single-source and register-tiled kernels.
+template <int P, int NWT, int W0, int NW, bool NT = false, bool SCALED = true>
 __device__ inline void load_range(const void *base, size_t vs, long s, typename PT<P>::calc *o) {
   typedef typename PT<P>::calc T;
 #pragma unroll
   for (int w = 0; w < NW; w++) Word<P>::template load<NT>(base, widx(s, W0 + w, NWT), o + w * PT<P>::per_word);
-  if constexpr (PT<P>::scaled) {
+  if constexpr (PT<P>::scaled && !SCALED) {
+#pragma unroll
+    for (int k = 0; k < NW * PT<P>::per_word; k++) keep_reg(o[k]);
+  }
+  if constexpr (PT<P>::scaled && SCALED) {
     T sc = site_scale<P>(base, vs, s, NWT);
 #pragma unroll
     for (int k = 0; k < NW * PT<P>::per_word; k++) o[k] *= sc;
@@ -166,9 +180,9 @@ __device__ inline void store_all(void *base, size_t vs, long s, const typename P
   }
 }
 
-template <int P, bool NT = false>
+template <int P, bool NT = false, bool SCALED = true>
 __device__ inline void load_spinor(const void *base, size_t vs, long s, typename PT<P>::calc *o) {
-  load_range<P, words_of<P>(kSpinorReals), 0, words_of<P>(kSpinorReals), NT>(base, vs, s, o);
+  load_range<P, words_of<P>(kSpinorReals), 0, words_of<P>(kSpinorReals), NT, SCALED>(base, vs, s, o);
 }
 template <int P, bool NT = false>
 __device__ inline void store_spinor(void *base, size_t vs, long s, const typename PT<P>::calc *v) {

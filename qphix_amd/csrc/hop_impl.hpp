@@ -92,6 +92,17 @@ template <bool C12, typename T> __device__ inline void make_link(const T *v, cx<
   }
 }
 
+// The link from unscaled stored rows v = U / su (FP16 storage, su the site
+// scale): returns U / su. A reconstructed third row is quadratic in the stored
+// rows, so it takes one factor su to come out on the same footing.
+template <bool C12, typename T> __device__ inline void make_link(const T *v, cx<T> (&U)[3][3], T su) {
+  make_link<C12>(v, U);
+  if constexpr (C12) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) U[2][c] = scale(su, U[2][c]);
+  }
+}
+
 template <int P, bool C12> struct GaugeLayout {
   static constexpr int link_reals = C12 ? 12 : 18;
   static constexpr int nw = words_of<P>(gauge_reals(C12));
@@ -149,7 +160,26 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
   const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
   T sp[kSpinorReals];
   cx<T> psi[4][3], U[3][3];
-  if constexpr (GL::per_link) {
+  if constexpr (PT<P>::scaled) {
+    // FP16 storage: links and spinors stay unscaled; the product of the link
+    // and neighbour-spinor site scales multiplies the term once, through its
+    // boundary phase (every term is linear in both)
+    static_assert(!GL::per_link, "FP16 links are read in pairs");
+    constexpr int nws = words_of<P>(kSpinorReals);
+    T lv[2 * GL::link_reals];
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true, false>(a.u, a.vs, s, lv);
+    const T su = site_scale<P>(a.u, a.vs, s, GL::nw);
+    load_spinor<P, false, false>(psi_p, a.vs, sb, sp);
+    const T fb = su * site_scale<P>(psi_p, a.vs, sb, nws);
+    to_spinor(sp, psi);
+    make_link<C12>(lv, U, su);
+    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tr * fb, ti * fb, acc);
+    load_spinor<P, false, false>(psi_p, a.vs, sf, sp);
+    const T ff = su * site_scale<P>(psi_p, a.vs, sf, nws);
+    to_spinor(sp, psi);
+    make_link<C12>(lv + GL::link_reals, U, su);
+    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr * ff, ti * ff, acc);
+  } else if constexpr (GL::per_link) {
     T lv[GL::link_reals];
     // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
     load_spinor<P>(psi_p, a.vs, sb, sp);
@@ -236,15 +266,16 @@ __device__ inline void hop_dim_shfl(const HopArgs &a, const void *psi_p, const c
 }
 
 // --- clover blocks --------------------------------------------------------
-// packed Hermitian block b (36 reals) of a 72-real clover site
-template <int P, int B> __device__ inline void load_clover_block(const void *c, size_t vs, long s,
-                                                                 typename PT<P>::calc *o) {
+// packed Hermitian block b (36 reals) of a 72-real clover site (SCALED =
+// false: FP16 values unscaled)
+template <int P, int B, bool SCALED = true>
+__device__ inline void load_clover_block(const void *c, size_t vs, long s, typename PT<P>::calc *o) {
   constexpr int nw = words_of<P>(kCloverReals);
   if constexpr ((36 % PT<P>::per_word) == 0) {
-    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word, true>(c, vs, s, o);
+    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word, true, SCALED>(c, vs, s, o);
   } else {
     typename PT<P>::calc all[kCloverReals];
-    load_range<P, nw, 0, nw, true>(c, vs, s, all);
+    load_range<P, nw, 0, nw, true, SCALED>(c, vs, s, all);
 #pragma unroll
     for (int k = 0; k < 36; k++) o[k] = all[36 * B + k];
   }
@@ -371,21 +402,32 @@ __device__ inline SiteNbrs site_nbrs(const HopArgs &a, unsigned k) {
 // One direction mu of the hop for R right-hand sides: each link is loaded and
 // rebuilt once and multiplies the projected neighbours of all R sources (the
 // 3 x (2 R) block contraction of a multi-RHS Dslash).
+// FP16 storage: U and the spinors are unscaled, su is the link site scale and
+// each term takes su times its spinor's site scale through the phase, as in
+// hop_dim (same expressions, same rounding)
 template <int P, bool C12, int ISIGN, int MU, bool FWD, int R>
 __device__ inline void hop_half_r(const cx<typename PT<P>::calc> (&U)[3][3], const HopArgs &a,
                                   const void *const (&psi)[R], unsigned sn, typename PT<P>::calc coef,
-                                  typename PT<P>::calc tr, typename PT<P>::calc ti,
+                                  typename PT<P>::calc tr, typename PT<P>::calc ti, typename PT<P>::calc su,
                                   cx<typename PT<P>::calc> (&acc)[R][4][3]) {
   typedef typename PT<P>::calc T;
 #pragma unroll
   for (int j = 0; j < R; j++) {
     T sp[kSpinorReals];
     cx<T> ps[4][3];
-    load_spinor<P>(psi[j], a.vs, sn, sp);
-    to_spinor(sp, ps);
-    apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12>(U, ps, coef, tr, ti, acc[j]);
+    if constexpr (PT<P>::scaled) {
+      load_spinor<P, false, false>(psi[j], a.vs, sn, sp);
+      const T f = su * site_scale<P>(psi[j], a.vs, sn, words_of<P>(kSpinorReals));
+      to_spinor(sp, ps);
+      apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12>(U, ps, coef, tr * f, ti * f, acc[j]);
+    } else {
+      load_spinor<P>(psi[j], a.vs, sn, sp);
+      to_spinor(sp, ps);
+      apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12>(U, ps, coef, tr, ti, acc[j]);
+    }
   }
 }
+
 
 template <int P, bool C12, int ISIGN, int MU, int R>
 __device__ inline void hop_dim_r(const HopArgs &a, const void *const (&psi)[R], unsigned s, unsigned sb, unsigned sf,
@@ -397,22 +439,31 @@ __device__ inline void hop_dim_r(const HopArgs &a, const void *const (&psi)[R], 
   const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
   cx<T> U[3][3];
   if constexpr (GL::per_link) {
+    static_assert(!PT<P>::scaled, "FP16 links are read in pairs");
     T lv[GL::link_reals];
     // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
     load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     make_link<C12>(lv, U);
-    hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, acc);
+    hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, (T)1, acc);
     // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
     load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     make_link<C12>(lv, U);
-    hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, acc);
+    hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, (T)1, acc);
   } else {
     T lv[2 * GL::link_reals];
-    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
-    make_link<C12>(lv, U);
-    hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, acc);
-    make_link<C12>(lv + GL::link_reals, U);
-    hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, acc);
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true, !PT<P>::scaled>(a.u, a.vs, s, lv);
+    if constexpr (PT<P>::scaled) {
+      const T su = site_scale<P>(a.u, a.vs, s, GL::nw);
+      make_link<C12>(lv, U, su);
+      hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, su, acc);
+      make_link<C12>(lv + GL::link_reals, U, su);
+      hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, su, acc);
+    } else {
+      make_link<C12>(lv, U);
+      hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, (T)1, acc);
+      make_link<C12>(lv + GL::link_reals, U);
+      hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, (T)1, acc);
+    }
   }
 }
 
@@ -425,24 +476,35 @@ __device__ inline void hop_term_r(const HopArgs &a, const void *const (&psi)[R],
   const bool tbc = (a.tbc_mask >> MU) & 1;
   const T tr = tbc ? (T)a.tbc_r[MU] : (T)1, ti = tbc ? (T)a.tbc_i[MU] : (T)0;
   cx<T> U[3][3];
+  T su = (T)1;
   if constexpr (GL::per_link) {
+    static_assert(!PT<P>::scaled, "FP16 links are read in pairs");
     T lv[GL::link_reals];
     load_range<P, GL::nw, (2 * MU + (FWD ? 1 : 0)) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     make_link<C12>(lv, U);
   } else {
     T lv[2 * GL::link_reals];
-    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
-    make_link<C12>(lv + (FWD ? GL::link_reals : 0), U);
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true, !PT<P>::scaled>(a.u, a.vs, s, lv);
+    if constexpr (PT<P>::scaled) {
+      su = site_scale<P>(a.u, a.vs, s, GL::nw);
+      make_link<C12>(lv + (FWD ? GL::link_reals : 0), U, su);
+    } else {
+      make_link<C12>(lv + (FWD ? GL::link_reals : 0), U);
+    }
   }
-  hop_half_r<P, C12, ISIGN, MU, FWD, R>(U, a, psi, sn, coef, tr, ti, acc);
+  hop_half_r<P, C12, ISIGN, MU, FWD, R>(U, a, psi, sn, coef, tr, ti, su, acc);
 }
 
-// packed Hermitian clover block B applied to R spinors (block loaded once)
+// packed Hermitian clover block B applied to R spinors (block loaded once).
+// FP16 storage: the unscaled block is applied and its site scale multiplies
+// the six outputs instead of the 36 stored reals.
 template <int P, int B, int R, typename T>
 __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s, const cx<T> (&in)[R][4][3],
                                             cx<T> (&out)[R][4][3]) {
+  constexpr bool raw = PT<P>::scaled;
   T m[36];
-  load_clover_block<P, B>(clov, vs, s, m);
+  load_clover_block<P, B, !raw>(clov, vs, s, m);
+  const T sc = raw ? site_scale<P>(clov, vs, s, words_of<P>(kCloverReals)) : (T)1;
 #pragma unroll
   for (int j = 0; j < R; j++) {
     cx<T> v[6], w[6];
@@ -450,17 +512,19 @@ __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s,
     for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
     herm6(m, v, w);
 #pragma unroll
-    for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = w[i];
+    for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = raw ? scale(sc, w[i]) : w[i];
   }
 }
 
 template <int P, int B, int R, typename T>
 __device__ inline void full_block_apply_r(const void *fcl, size_t vs, long s, const cx<T> (&in)[R][4][3],
                                           cx<T> (&out)[R][4][3]) {
+  constexpr bool raw = PT<P>::scaled;
   constexpr int nw = words_of<P>(kFullCloverReals);
   constexpr int bw = nw / 2;
   T m[72];
-  load_range<P, nw, B * bw, bw, true>(fcl, vs, s, m);
+  load_range<P, nw, B * bw, bw, true, !raw>(fcl, vs, s, m);
+  const T sc = raw ? site_scale<P>(fcl, vs, s, nw) : (T)1;
 #pragma unroll
   for (int j = 0; j < R; j++) {
     cx<T> v[6], w[6];
@@ -468,7 +532,7 @@ __device__ inline void full_block_apply_r(const void *fcl, size_t vs, long s, co
     for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
     full6(m, v, w);
 #pragma unroll
-    for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = w[i];
+    for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = raw ? scale(sc, w[i]) : w[i];
   }
 }
 
