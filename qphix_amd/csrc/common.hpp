@@ -207,6 +207,24 @@ template <typename T> __device__ inline cx<T> cmulc(cx<T> a, cx<T> b) {  // conj
 template <typename T> __device__ inline cx<T> scale(T s, cx<T> a) { return mk(s * a.r, s * a.i); }
 template <typename T> __device__ inline cx<T> conj(cx<T> a) { return mk(a.r, -a.i); }
 
+// Complex products and multiply-adds as explicit FMA chains (one rounding per
+// FMA, product terms never rounded on their own): a b, conj(a) b, s + a b and
+// s + conj(a) b.
+__device__ inline float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename T> __device__ inline cx<T> cmul_f(cx<T> a, cx<T> b) {
+  return mk(fmat(-a.i, b.i, a.r * b.r), fmat(a.i, b.r, a.r * b.i));
+}
+template <typename T> __device__ inline cx<T> cmulc_f(cx<T> a, cx<T> b) {
+  return mk(fmat(a.i, b.i, a.r * b.r), fmat(-a.i, b.r, a.r * b.i));
+}
+template <typename T> __device__ inline cx<T> cfma(cx<T> a, cx<T> b, cx<T> s) {
+  return mk(fmat(-a.i, b.i, fmat(a.r, b.r, s.r)), fmat(a.i, b.r, fmat(a.r, b.i, s.i)));
+}
+template <typename T> __device__ inline cx<T> cfmac(cx<T> a, cx<T> b, cx<T> s) {
+  return mk(fmat(a.i, b.i, fmat(a.r, b.r, s.r)), fmat(-a.i, b.r, fmat(a.r, b.i, s.i)));
+}
+
 // multiply by a compile-time phase in {1, -1, i, -i} given as (PR, PI)
 template <int PR, int PI, typename T> __device__ inline cx<T> mulph(cx<T> a) {
   if constexpr (PI == 0) {

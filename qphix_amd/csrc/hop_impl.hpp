@@ -115,7 +115,8 @@ template <int P, bool C12> struct GaugeLayout {
 // the dimension has none: a branch here would split the kernel into basic
 // blocks and stop the scheduler from issuing the next hop's loads early.
 // apply_hop_h takes the projected half spinor h = (1 -/+ isign g_mu) psi.
-template <int MU, int S, bool FWD, bool C12, typename T>
+// FMA: the link products as FMA chains (cmul_f / cfma), used for FP16 storage
+template <int MU, int S, bool FWD, bool C12, bool FMA = false, typename T>
 __device__ inline void apply_hop_h(const cx<T> (&U)[3][3], const cx<T> (&h)[2][3], T coef, T tr, T ti,
                                    cx<T> (&acc)[4][3]) {
   cx<T> uh[2][3];
@@ -124,7 +125,15 @@ __device__ inline void apply_hop_h(const cx<T> (&U)[3][3], const cx<T> (&h)[2][3
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       cx<T> sum;
-      if constexpr (FWD) {
+      if constexpr (FMA && FWD) {
+        sum = cmul_f(U[i][0], h[r][0]);
+        sum = cfma(U[i][1], h[r][1], sum);
+        sum = cfma(U[i][2], h[r][2], sum);
+      } else if constexpr (FMA) {
+        sum = cmulc_f(U[0][i], h[r][0]);
+        sum = cfmac(U[1][i], h[r][1], sum);
+        sum = cfmac(U[2][i], h[r][2], sum);
+      } else if constexpr (FWD) {
         sum = U[i][0] * h[r][0];
         sum = sum + U[i][1] * h[r][1];
         sum = sum + U[i][2] * h[r][2];
@@ -143,12 +152,12 @@ __device__ inline void apply_hop_h(const cx<T> (&U)[3][3], const cx<T> (&h)[2][3
   recon_acc<MU, S>(acc, uh, coef);
 }
 
-template <int MU, int S, bool FWD, bool C12, typename T>
+template <int MU, int S, bool FWD, bool C12, bool FMA = false, typename T>
 __device__ inline void apply_hop(const cx<T> (&U)[3][3], const cx<T> (&psi)[4][3], T coef, T tr, T ti,
                                  cx<T> (&acc)[4][3]) {
   cx<T> h[2][3];
   project<MU, S>(psi, h);
-  apply_hop_h<MU, S, FWD, C12>(U, h, coef, tr, ti, acc);
+  apply_hop_h<MU, S, FWD, C12, FMA>(U, h, coef, tr, ti, acc);
 }
 
 template <int P, bool C12, int ISIGN, int MU>
@@ -173,12 +182,12 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
     const T fb = su * site_scale<P>(psi_p, a.vs, sb, nws);
     to_spinor(sp, psi);
     make_link<C12>(lv, U, su);
-    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tr * fb, ti * fb, acc);
+    apply_hop<MU, ISIGN, false, C12, true>(U, psi, cb_coef, tr * fb, ti * fb, acc);
     load_spinor<P, false, false>(psi_p, a.vs, sf, sp);
     const T ff = su * site_scale<P>(psi_p, a.vs, sf, nws);
     to_spinor(sp, psi);
     make_link<C12>(lv + GL::link_reals, U, su);
-    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr * ff, ti * ff, acc);
+    apply_hop<MU, -ISIGN, true, C12, true>(U, psi, cf_coef, tr * ff, ti * ff, acc);
   } else if constexpr (GL::per_link) {
     T lv[GL::link_reals];
     // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
@@ -282,7 +291,9 @@ __device__ inline void load_clover_block(const void *c, size_t vs, long s, typen
 }
 
 // out = M in for a packed Hermitian 6x6 block m = {diag[6], offd[15][2]}
-template <typename T> __device__ inline void herm6(const T *m, const cx<T> (&in)[6], cx<T> (&out)[6]) {
+// (FMA: the multiply-adds as FMA chains, used for FP16 storage)
+template <bool FMA = false, typename T>
+__device__ inline void herm6(const T *m, const cx<T> (&in)[6], cx<T> (&out)[6]) {
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     cx<T> s = scale(m[i], in[i]);
@@ -290,22 +301,30 @@ template <typename T> __device__ inline void herm6(const T *m, const cx<T> (&in)
     for (int j = 0; j < 6; j++) {
       if (j < i) {
         const int e = i * (i - 1) / 2 + j;
-        s = s + mk(m[6 + 2 * e], m[6 + 2 * e + 1]) * in[j];
+        if constexpr (FMA) s = cfma(mk(m[6 + 2 * e], m[6 + 2 * e + 1]), in[j], s);
+        else s = s + mk(m[6 + 2 * e], m[6 + 2 * e + 1]) * in[j];
       } else if (j > i) {
         const int e = j * (j - 1) / 2 + i;
-        s = s + cmulc(mk(m[6 + 2 * e], m[6 + 2 * e + 1]), in[j]);
+        if constexpr (FMA) s = cfmac(mk(m[6 + 2 * e], m[6 + 2 * e + 1]), in[j], s);
+        else s = s + cmulc(mk(m[6 + 2 * e], m[6 + 2 * e + 1]), in[j]);
       }
     }
     out[i] = s;
   }
 }
 
-template <typename T> __device__ inline void full6(const T *m, const cx<T> (&in)[6], cx<T> (&out)[6]) {
+template <bool FMA = false, typename T>
+__device__ inline void full6(const T *m, const cx<T> (&in)[6], cx<T> (&out)[6]) {
 #pragma unroll
   for (int i = 0; i < 6; i++) {
-    cx<T> s = mk(m[12 * i], m[12 * i + 1]) * in[0];
+    cx<T> s;
+    if constexpr (FMA) s = cmul_f(mk(m[12 * i], m[12 * i + 1]), in[0]);
+    else s = mk(m[12 * i], m[12 * i + 1]) * in[0];
 #pragma unroll
-    for (int j = 1; j < 6; j++) s = s + mk(m[12 * i + 2 * j], m[12 * i + 2 * j + 1]) * in[j];
+    for (int j = 1; j < 6; j++) {
+      if constexpr (FMA) s = cfma(mk(m[12 * i + 2 * j], m[12 * i + 2 * j + 1]), in[j], s);
+      else s = s + mk(m[12 * i + 2 * j], m[12 * i + 2 * j + 1]) * in[j];
+    }
     out[i] = s;
   }
 }
@@ -419,7 +438,7 @@ __device__ inline void hop_half_r(const cx<typename PT<P>::calc> (&U)[3][3], con
       load_spinor<P, false, false>(psi[j], a.vs, sn, sp);
       const T f = su * site_scale<P>(psi[j], a.vs, sn, words_of<P>(kSpinorReals));
       to_spinor(sp, ps);
-      apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12>(U, ps, coef, tr * f, ti * f, acc[j]);
+      apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12, true>(U, ps, coef, tr * f, ti * f, acc[j]);
     } else {
       load_spinor<P>(psi[j], a.vs, sn, sp);
       to_spinor(sp, ps);
@@ -498,7 +517,7 @@ __device__ inline void hop_term_r(const HopArgs &a, const void *const (&psi)[R],
 // packed Hermitian clover block B applied to R spinors (block loaded once).
 // FP16 storage: the unscaled block is applied and its site scale multiplies
 // the six outputs instead of the 36 stored reals.
-template <int P, int B, int R, typename T>
+template <int P, int B, int R, bool FMA = PT<P>::scaled, typename T>
 __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s, const cx<T> (&in)[R][4][3],
                                             cx<T> (&out)[R][4][3]) {
   constexpr bool raw = PT<P>::scaled;
@@ -510,7 +529,7 @@ __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s,
     cx<T> v[6], w[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
-    herm6(m, v, w);
+    herm6<FMA>(m, v, w);
 #pragma unroll
     for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = raw ? scale(sc, w[i]) : w[i];
   }
@@ -530,7 +549,7 @@ __device__ inline void full_block_apply_r(const void *fcl, size_t vs, long s, co
     cx<T> v[6], w[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
-    full6(m, v, w);
+    full6<raw>(m, v, w);
 #pragma unroll
     for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = raw ? scale(sc, w[i]) : w[i];
   }
@@ -559,8 +578,8 @@ __device__ inline void hop_init_r(const HopArgs &a, unsigned s, const void *cons
 #pragma unroll
           for (int c = 0; c < 3; c++) acc[j][sp][c] = scale(al, ch[j][sp][c]);
     } else if constexpr (KIND == KA_CLOVER) {
-      clover_block_apply_r<P, 0, R>(a.clov, a.vs, s, ch, acc);
-      clover_block_apply_r<P, 1, R>(a.clov, a.vs, s, ch, acc);
+      clover_block_apply_r<P, 0, R, false>(a.clov, a.vs, s, ch, acc);
+      clover_block_apply_r<P, 1, R, false>(a.clov, a.vs, s, ch, acc);
 #pragma unroll
       for (int j = 0; j < R; j++) add_ig5((T)(ISIGN * a.rho), ch[j], acc[j]);
     } else {
@@ -704,10 +723,10 @@ __device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, voi
 }
 
 // minimum waves per SIMD the register allocation must allow for the hop
-// kernels: the FP16 forms would otherwise take ~170 VGPRs (two waves per SIMD);
-// 168 gives three without spilling
+// kernels: four for the FP16 forms (at most 128 VGPRs; the issue-bound FP16
+// kernels need the waves to hide their load latency)
 #ifndef QPHIX_HOP_WAVES16
-#define QPHIX_HOP_WAVES16 3
+#define QPHIX_HOP_WAVES16 4
 #endif
 constexpr int hop_min_waves(int p) { return p == P16 ? QPHIX_HOP_WAVES16 : 1; }
 
