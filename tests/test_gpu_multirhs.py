@@ -163,3 +163,65 @@ def test_register_tiled_multi_rhs(forced_rhs_tiling, reg, rt, prec, c12):
                    prec_mass_rho=0.03, t_boundary=-1.0)
     for isign in (1, -1):
         check(lambda o: M.apply_multi(o, psi, isign, 1), lambda o, r: M(o, psi[r], isign, 1), 1)
+
+
+@pytest.mark.parametrize("reg,rt", [(2, 2), (4, 4)])
+def test_f16_subnormal_halves_tile_equals_single(forced_rhs_tiling, reg, rt):
+    """FP16 storage with components far below their site's largest one (1e-6 of
+    it, subnormal halves once normalised by the per-site scale): the register
+    tiles still give every source its single-source result bit for bit, and
+    both agree with the oracle on the values the device holds. Such halves are
+    planted on purpose: how their half-to-single conversions are compiled is
+    where the single-source and tiled kernels could part."""
+    L = [4, 4, 4, 8]
+    n = 5
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    u = oracle.gen_gauge(L, 35, 0.4)
+    clov = oracle.clover_build(L, u, 1.0, 0.05)
+    inv_h = oracle.clover_invert(clov[:vh * 72])
+    g = q.Geometry(L, q.F16, True)
+    ud = q.pack_gauge(g, u)
+    inv = g.clover().upload(inv_h)
+    A = g.clover().upload(clov[vh * 72:])
+    rng = np.random.default_rng(7)
+
+    def planted(seed, cb):
+        h = oracle.gen_spinor(L, seed, cb).reshape(vh, 24)
+        h[rng.random((vh, 24)) < 0.3] *= 1e-6
+        return h.ravel()
+
+    psi = [g.spinor().upload(planted(90 + r, 1), 1) for r in range(n)]
+    chi = [g.spinor().upload(planted(100 + r, 0), 0) for r in range(n)]
+    held = psi[0].download(1)
+    tiny = np.abs(held.reshape(vh, 24)) / np.abs(held.reshape(vh, 24)).max(axis=1, keepdims=True)
+    assert np.count_nonzero((tiny > 0) & (tiny < 2.0 ** -14)) > 100  # subnormal halves are stored
+    q.call("qphix_tune_rhs", reg, rt)
+    D = q.Dslash(g, t_boundary=-1.0)
+    hp = oracle.hop_params(-1.0)
+    for isign in (1, -1):
+        for kind in ("clover", "achimbdpsi", "wilson"):
+            outs = [g.spinor() for _ in range(n)]
+            if kind == "clover":
+                D.clov_dslash_multi(outs, psi, ud[0], inv, isign, 0)
+            elif kind == "achimbdpsi":
+                D.clov_achimbdpsi_multi(outs, psi, chi, ud[0], A, 0.25, isign, 0)
+            else:
+                D.dslash_multi(outs, psi, ud[0], isign, 0)
+            for r in range(n):
+                one = g.spinor()
+                if kind == "clover":
+                    D.clov_dslash(one, psi[r], ud[0], inv, isign, 0)
+                elif kind == "achimbdpsi":
+                    D.clov_achimbdpsi(one, psi[r], chi[r], ud[0], A, 0.25, isign, 0)
+                else:
+                    D.dslash(one, psi[r], ud[0], isign, 0)
+                got = one.download(0)
+                assert np.array_equal(outs[r].download(0), got), (kind, isign, r)
+            hop = oracle.hop(L, u, psi[n - 1].download(1), isign, 0, hp)
+            if kind == "clover":
+                ref = oracle.clover_apply(inv_h, hop)
+            elif kind == "achimbdpsi":
+                ref = oracle.clover_apply(clov[vh * 72:], chi[n - 1].download(0)) - 0.25 * hop
+            else:
+                ref = hop
+            assert rel(got, ref) < 5e-3, (kind, isign, rel(got, ref))
