@@ -172,7 +172,8 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
   if constexpr (PT<P>::scaled) {
     // FP16 storage: links and spinors stay unscaled; the product of the link
     // and neighbour-spinor site scales multiplies the term once, through its
-    // boundary phase (every term is linear in both)
+    // boundary phase (every term is linear in both); the link products are
+    // FMA chains
     static_assert(!GL::per_link, "FP16 links are read in pairs");
     constexpr int nws = words_of<P>(kSpinorReals);
     T lv[2 * GL::link_reals];
@@ -222,7 +223,8 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
 // wave's own-index spinor of lane n - s0, so each lane projects its own-index
 // spinor with the direction's projector and the consumers read the 12-real
 // half spinor across lanes; lanes whose neighbour lies outside the wave load
-// it. The arithmetic is that of hop_dim (same projection, same products).
+// it. The arithmetic is that of hop_dim for FP32 storage (same projection,
+// same products); the kernels use this form for FP32 only (hop_shfl).
 template <typename T> __device__ inline void shfl_half(const cx<T> (&ho)[2][3], int src, cx<T> (&h)[2][3]) {
 #pragma unroll
   for (int r = 0; r < 2; r++)
