@@ -622,10 +622,14 @@ def batch_compare(q):
     ok_b = all(st.status == q.OK for st in stb)
     ok_s = all(st.status == q.OK for st in sts)
     worst = max(st.rsd_sq_final for st in stb) ** 0.5
-    log(f"batch tm-clover: batched {tb:.3f} s, one-by-one {ts:.3f} s, speedup {ts / tb:.2f}")
+    apps_b = sum(int(st.mv_apps) for st in stb)
+    apps_s = sum(int(st.mv_apps) for st in sts)
+    log(f"batch tm-clover: batched {tb:.3f} s, one-by-one {ts:.3f} s, speedup {ts / tb:.2f}; "
+        f"operator applications {apps_b} batched, {apps_s} one by one")
     return {"lattices": lats, "rhs_per_lattice": 4, "batched_seconds": round(tb, 4),
             "one_by_one_seconds": round(ts, 4), "speedup": round(ts / tb, 3),
-            "all_converged": ok_b and ok_s, "worst_true_rel_resid_batched": worst}
+            "all_converged": ok_b and ok_s, "worst_true_rel_resid_batched": worst,
+            "operator_applications_batched": apps_b, "operator_applications_one_by_one": apps_s}
 
 
 if __name__ == "__main__":
