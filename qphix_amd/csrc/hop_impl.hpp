@@ -795,15 +795,16 @@ struct HopSeg {
   int sys;   // first system of the segment
   int nsys;  // tiled form: systems sys .. sys + nsys - 1 share the operator (a.psis / outs / chis)
 };
-// Occupancy ceiling of the batched hops. The FP16 forms (118-169 VGPRs) sit
-// near the three-waves-per-SIMD line, where the scheduler trades load overlap
-// for registers: the flat loads of the batched kernels then end up waited on
-// one by one (the register-tiled A chi - beta D psi form ran 63% slower).
-// Two waves leave it the registers to keep the loads in flight.
-constexpr int batch_max_waves(int p) { return p == P16 ? 2 : 8; }
+// Occupancy ceiling of the batched hops. The FP16 A chi - beta D psi forms
+// (146-169 VGPRs) sit near the three-waves-per-SIMD line, where the scheduler
+// trades load overlap for registers: their flat loads then end up waited on
+// one by one (the register-tiled form ran 63% slower). Two waves leave them
+// the registers to keep the loads in flight; the other kinds keep theirs
+// (the full-block A^-1 tile runs 15% faster at three waves).
+constexpr int batch_max_waves(int p, int kind) { return p == P16 && kind == KA_CLOVER ? 2 : 8; }
 
 template <int P, bool C12, int ISIGN, int KIND>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, batch_max_waves(P))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, batch_max_waves(P, KIND))))
 hop_batch_kernel(const HopSeg *segs, const int *blk_off, int nseg,
                                                          const int *active) {
   const int total = blk_off[nseg];
@@ -822,7 +823,7 @@ hop_batch_kernel(const HopSeg *segs, const int *blk_off, int nseg,
 // every work-item applies the links and clover of its site to all R sources
 // (hop_body_r). A segment is skipped once all its systems have converged.
 template <int P, bool C12, int ISIGN, int KIND, int R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, batch_max_waves(P))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, batch_max_waves(P, KIND))))
 hop_batch_tile_kernel(const HopSeg *segs, const int *blk_off, int nseg,
                                                               const int *active) {
   const int total = blk_off[nseg];
