@@ -120,7 +120,9 @@ class InvCG : public detail::NativeSolverBase<FT, veclen, soalen, compress12, Ev
   }
 
  protected:
-  // CG on M^+M for a user-defined operator (invcg.h:88-456)
+  // CG on M^+M for a user-defined operator (invcg.h:88-456), with the
+  // reference's counters (site_flops at invcg.h:180,224,308,337,416, mv_apps
+  // at :210,294); rsd_sq_final on convergence is the updated |r|^2
   void generic(Spinor *const x[num_flav], const Spinor *const rhs[num_flav], double RsdTarget, int &n_iters,
                double &rsd_sq_final, unsigned long &site_flops, unsigned long &mv_apps, int isign, bool verbose,
                int cb, ResiduumType rt) const override
@@ -139,6 +141,7 @@ class InvCG : public detail::NativeSolverBase<FT, veclen, soalen, compress12, Ev
       }
       return s;
     };
+    const unsigned long sf = 12ul * num_flav;
     site_flops = 0;
     mv_apps = 0;
     double chi_sq = 0, t;
@@ -146,6 +149,7 @@ class InvCG : public detail::NativeSolverBase<FT, veclen, soalen, compress12, Ev
       detail::check(qphix_norm2_spinor(gh, rhs[f], &t), "norm2");
       chi_sq += t;
     }
+    site_flops += 4 * sf;
     double rsd_sq = RsdTarget * RsdTarget * (rt == RELATIVE ? chi_sq : 1.0);
     M(mp, x, isign);
     M(mmp, mp, -isign);
@@ -155,6 +159,7 @@ class InvCG : public detail::NativeSolverBase<FT, veclen, soalen, compress12, Ev
       detail::check(qphix_xmy_norm2(gh, r[f], rhs[f], mmp[f], &t), "xmyNorm2");
       cp += t;
     }
+    site_flops += 6 * sf;
     if (cp <= rsd_sq) {
       n_iters = 0;
       rsd_sq_final = cp / chi_sq;
@@ -168,28 +173,23 @@ class InvCG : public detail::NativeSolverBase<FT, veclen, soalen, compress12, Ev
       M(mmp, mp, -isign);
       mv_apps += 2;
       double d = norm2(mp), a = c / d;
+      site_flops += 4 * sf;
       cp = 0;
       for (int f = 0; f < num_flav; f++) {
         detail::check(qphix_rmammp_norm2_rxpap(gh, r[f], a, mmp[f], x[f], p[f], &t), "rmammp");
         cp += t;
       }
+      site_flops += 12 * sf;
       if (verbose) std::printf("CG: iter %d: r = %e target = %e\n", k, cp, rsd_sq);
       if (cp < rsd_sq) {
         n_iters = k;
-        M(mp, x, isign);
-        M(mmp, mp, -isign);
-        mv_apps += 2;
-        cp = 0;
-        for (int f = 0; f < num_flav; f++) {
-          detail::check(qphix_xmy_norm2(gh, r[f], rhs[f], mmp[f], &t), "xmyNorm2");
-          cp += t;
-        }
         rsd_sq_final = cp;
         for (Spinor *s : w) g.free(s);
         return;
       }
       double b = cp / c;
       for (int f = 0; f < num_flav; f++) detail::check(qphix_aypx(gh, b, r[f], p[f]), "aypx");
+      site_flops += 4 * sf;
     }
     n_iters = max_iters_;
     rsd_sq_final = cp;

@@ -513,6 +513,7 @@ class MInvCG:
         fl = C.c_ulong()
         call("qphix_minvcg_solve", self.h, _pa(xs), _pa(rhs), ns, darr(shifts), darr(rsd_target),
              isign, int(verbose), target_cb, C.byref(nit), fin, C.byref(fl), C.byref(mv))
+        self.site_flops = fl.value
         return nit.value, list(fin), mv.value
 
 

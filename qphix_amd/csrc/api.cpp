@@ -1468,7 +1468,12 @@ extern "C" int qphix_solver_destroy(qphix_solver *s) {
   return QPHIX_OK;
 }
 
-// invcg.h:88-456
+// invcg.h:88-456.  Counters as the reference's default build (TIMING_CG not
+// defined) counts them: site_flops at invcg.h:180,224,308,337,416 and mv_apps
+// at :210,294, including the initial M^+M x, which is skipped (not its count)
+// when x is exactly zero.  On convergence the reference leaves rsd_sq_final
+// unset; here it is the recursively updated |r|^2 (absolute), the quantity the
+// convergence test used.
 static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                     int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -1486,11 +1491,13 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
     p.assign(s->work.begin() + 2 * nf, s->work.begin() + 3 * nf);
     r.assign(s->work.begin() + 3 * nf, s->work.begin() + 4 * nf);
   }
+  const unsigned long sf = 12ul * (unsigned long)nf;  // the reference counts site flops in units of 12 * num_flav
   st->site_flops = 0;
   st->mv_apps = 0;
   st->status = QPHIX_OK;
   double chi_sq;
   TRY(mf_norm2(g, nf, rhs, chi_sq));
+  st->site_flops += 4 * sf;
   double rsd_sq = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= chi_sq;
   double cp = 0;
@@ -1502,13 +1509,14 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
   } else {
     TRY(op_apply_v(M, mp, x, isign, cb));
     TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
-    st->mv_apps += 2;
     for (int f = 0; f < nf; f++) {
       double t;
       TRY(qphix_xmy_norm2(g, r[f], rhs[f], mmp[f], &t));
       cp += t;
     }
   }
+  st->mv_apps += 2;
+  st->site_flops += 6 * sf;
   if (verbose) printf("CG: ||r||^2 = %e target = %e\n", cp, rsd_sq);
   if (cp <= rsd_sq) {
     st->n_iters = 0;
@@ -1523,6 +1531,7 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
     st->mv_apps += 2;
     double d;
     TRY(mf_norm2(g, nf, cv(mp), d));
+    st->site_flops += 4 * sf;
     const double a = c / d;
     cp = 0;
     for (int f = 0; f < nf; f++) {
@@ -1530,23 +1539,16 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
       TRY(qphix_rmammp_norm2_rxpap(g, r[f], a, mmp[f], x[f], p[f], &t));
       cp += t;
     }
+    st->site_flops += 12 * sf;
     if (verbose) printf("CG: iter %d: r = %e target = %e\n", k, cp, rsd_sq);
     if (cp < rsd_sq) {
       st->n_iters = k;
-      TRY(op_apply_v(M, mp, x, isign, cb));
-      TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
-      st->mv_apps += 2;
-      cp = 0;
-      for (int f = 0; f < nf; f++) {
-        double t;
-        TRY(qphix_xmy_norm2(g, r[f], rhs[f], mmp[f], &t));
-        cp += t;
-      }
       st->rsd_sq_final = cp;
       return QPHIX_OK;
     }
     const double b = cp / c;
     for (int f = 0; f < nf; f++) TRY(qphix_aypx(g, b, r[f], p[f]));
+    st->site_flops += 4 * sf;
   }
   st->n_iters = s->max_iters;
   st->rsd_sq_final = cp;
@@ -1554,7 +1556,9 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
   return QPHIX_OK;
 }
 
-// invbicgstab.h:86-262
+// invbicgstab.h:86-262.  Counters as invbicgstab.h:158-240 counts them,
+// including the initial M x (skipped, not its count, when x is exactly zero);
+// the |r0|^2 of the iteration-0 test is not counted there either.
 static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                           int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -1571,11 +1575,13 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
     SV *sets[5] = {&r0, &r, &p, &v, &t};
     for (int k = 0; k < 5; k++) sets[k]->assign(s->work.begin() + k * nf, s->work.begin() + (k + 1) * nf);
   }
+  const unsigned long sf = 12ul * (unsigned long)nf;  // the reference counts site flops in units of 12 * num_flav
   st->site_flops = 0;
   st->mv_apps = 0;
   st->status = QPHIX_OK;
   double rhs_sq;
   TRY(mf_norm2(g, nf, rhs, rhs_sq));
+  st->site_flops += 4 * sf;
   double rsd_sq = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_sq *= rhs_sq;
   double r_norm;
@@ -1586,10 +1592,11 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
     r_norm = rhs_sq;
   } else {
     TRY(op_apply_v(M, r0, x, isign, cb));
-    st->mv_apps++;
     for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r0[f]));
     TRY(mf_norm2(g, nf, cv(r0), r_norm));
   }
+  st->mv_apps++;
+  st->site_flops += 2 * sf;
   if (r_norm < rsd_sq) {
     st->rsd_sq_final = r_norm;
     st->n_iters = 0;
@@ -1610,6 +1617,7 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
   for (k = 1; k <= s->max_iters && notconv; k++) {
     rho[0] = r0r[0];
     rho[1] = r0r[1];
+    st->site_flops += 8 * sf;
     if (rho[0] == 0 && rho[1] == 0) {
       st->rsd_sq_final = r_norm;
       st->n_iters = k;
@@ -1621,10 +1629,12 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
     cdiv(t2, alpha, omega);
     cmul(beta, t1, t2);
     for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_p_update(g, r[f], p[f], v[f], beta, omega));
+    st->site_flops += 16 * sf;
     TRY(op_apply_v(M, v, cv(p), isign, cb));
     st->mv_apps++;
     double tmp[2];
     TRY(mf_cdot(g, nf, cv(r0), cv(v), tmp));
+    st->site_flops += 8 * sf;
     if (tmp[0] == 0 && tmp[1] == 0) {
       st->n_iters = k;
       st->rsd_sq_final = r_norm;
@@ -1635,10 +1645,12 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
     rho_prev[0] = rho[0];
     rho_prev[1] = rho[1];
     for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_s_update(g, alpha, r[f], v[f]));
+    st->site_flops += 8 * sf;
     TRY(op_apply_v(M, t, cv(r), isign, cb));
     st->mv_apps++;
     double tr[3];  // <t, r> and |t|^2 in one sweep
     TRY(mf_cdotnorm(g, nf, cv(t), cv(r), tr));
+    st->site_flops += 4 * sf;
     const double t_norm = tr[2];
     if (t_norm == 0) {
       st->rsd_sq_final = r_norm;
@@ -1646,10 +1658,12 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
       st->status = QPHIX_EBREAKDOWN;
       return QPHIX_OK;
     }
+    st->site_flops += 8 * sf;
     omega[0] = tr[0] / t_norm;
     omega[1] = tr[1] / t_norm;
     double ru[3];
     TRY(mf_rxupdate_cdot(g, nf, x, r.data(), cv(t), cv(p), cv(r0), omega, alpha, ru));
+    st->site_flops += 28 * sf;
     r_norm = ru[0];
     r0r[0] = ru[1];
     r0r[1] = ru[2];
@@ -1663,7 +1677,8 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
 }
 
 // invmr.h:71-221 (InvMRBase::operator(); InvMR stops on the residual,
-// InvMRSmoother runs max_iters iterations and reports rsd_sq_final = -1)
+// InvMRSmoother runs max_iters iterations and reports rsd_sq_final = -1);
+// counters as invmr.h:95-182 counts them, the initial M x included
 static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                     int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -1677,6 +1692,7 @@ static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, dou
     mr.assign(s->work.begin(), s->work.begin() + nf);
     r.assign(s->work.begin() + nf, s->work.begin() + 2 * nf);
   }
+  const unsigned long sf = 12ul * (unsigned long)nf;  // the reference counts site flops in units of 12 * num_flav
   st->site_flops = 0;
   st->mv_apps = 0;
   st->status = QPHIX_OK;
@@ -1692,15 +1708,17 @@ static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, dou
     for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r[f], rhs[f]));
   } else {
     TRY(op_apply_v(M, r, x, isign, cb));
-    st->mv_apps++;
     for (int f = 0; f < nf; f++) TRY(qphix_bicgstab_xmy(g, rhs[f], r[f]));
   }
+  st->mv_apps++;
+  st->site_flops += 2 * sf;
   const bool term = s->mr_terminate != 0;
   double norm_chi = 0, rsd_sq = 0, cp = 0;
   if (term) {
     TRY(mf_norm2(g, nf, rhs, norm_chi));
     rsd_sq = rsd_target * rsd_target * (rtype == QPHIX_RESID_RELATIVE ? norm_chi : 1.0);
     TRY(mf_norm2(g, nf, cv(r), cp));
+    st->site_flops += 8 * sf;
     if (verbose) printf("QPhiX MR: iter=0: || r ||^2 = %16.8e Target || r ||^2 = %16.8e\n", cp, rsd_sq);
     if (cp < rsd_sq) {
       st->n_iters = 0;
@@ -1716,10 +1734,13 @@ static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, dou
     st->mv_apps++;
     double cd[3];  // <Mr, r>, |Mr|^2
     TRY(mf_cdotnorm(g, nf, cv(mr), cv(r), cd));
+    st->site_flops += 12 * sf;
     const double a[2] = {s->omega * cd[0] / cd[2], s->omega * cd[1] / cd[2]};
     for (int f = 0; f < nf; f++) TRY(qphix_mr_rxupdate(g, x[f], r[f], mr[f], a));
+    st->site_flops += 16 * sf;
     if (term) {
       TRY(mf_norm2(g, nf, cv(r), cp));
+      st->site_flops += 4 * sf;
       if (verbose) printf("QPhiX MR: iter=%d: || r ||^2 = %16.8e Target || r ||^2 = %16.8e\n", k, cp, rsd_sq);
       more = k < s->max_iters && cp > rsd_sq;
     } else {
@@ -1736,7 +1757,10 @@ static int solve_mr(qphix_solver *s, void *const *x, const void *const *rhs, dou
   return QPHIX_OK;
 }
 
-// inv_richardson_multiprec.h:42-239
+// inv_richardson_multiprec.h:42-239; counters as :55-228 counts them (outer
+// site flops 47 per norm, 71 per x - y norm, 96 per x/r update, plus the inner
+// solver's site flops times num_flav; the initial M x is counted also when x
+// is exactly zero and its product is skipped)
 static int solve_richardson(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                             int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -1755,10 +1779,11 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
     SV *sets[6] = {&r, &dx, &tmp, &tmp2, &ri, &dxi};
     for (int k = 0; k < 6; k++) sets[k]->assign(s->work.begin() + k * nf, s->work.begin() + (k + 1) * nf);
   }
-  unsigned long mv_outer = 0, mv_inner = 0, fl_inner = 0;
+  unsigned long mv_outer = 0, mv_inner = 0, fl_inner = 0, fl_outer = 0;
   st->status = QPHIX_OK;
   double rhs_sq, r_norm_sq;
   TRY(mf_norm2(g, nf, rhs, rhs_sq));
+  fl_outer += 24 + 23;
   double rsd_t = rsd_target * rsd_target;
   if (rtype == QPHIX_RESID_RELATIVE) rsd_t *= rhs_sq;
   auto apply_outer = [&](const void *const *in) -> int {
@@ -1780,6 +1805,7 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
       TRY(qphix_xmy_norm2(g, r[f], rhs[f], tmp[f], &t));
       r_norm_sq += t;
     }
+    fl_outer += 24 + 24 + 23;
     return QPHIX_OK;
   };
   bool xz;
@@ -1787,6 +1813,8 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
   if (xz) {  // r = rhs - M 0
     for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, r[f], rhs[f]));
     r_norm_sq = rhs_sq;
+    mv_outer += s->mdagm ? 2 : 1;
+    fl_outer += 24 + 24 + 23;
   } else {
     TRY(true_resid());
   }
@@ -1795,7 +1823,7 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
     st->n_iters = 0;
     st->rsd_sq_final = r_norm_sq;
     st->mv_apps = mv_outer;
-    st->site_flops = 0;
+    st->site_flops = fl_outer;
     return QPHIX_OK;
   }
   int iter;
@@ -1822,6 +1850,7 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
       TRY(qphix_richardson_rxupdate_norm_r(g, x[f], r[f], dx[f], tmp[f], &t));
       r_norm_sq += t;
     }
+    fl_outer += 8 * 12;
     if (verbose)
       printf("RICHARDSON: Iter %d inner iters %d r_norm_sq = %e Target = %e\n", iter, ist.n_iters, r_norm_sq, rsd_t);
     if (r_norm_sq < rsd_t) {
@@ -1829,7 +1858,7 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
       st->rsd_sq_final = r_norm_sq / rhs_sq;
       st->n_iters = iter;
       st->mv_apps = mv_outer + mv_inner;
-      st->site_flops = fl_inner * nf;
+      st->site_flops = fl_outer + fl_inner * nf;
       return QPHIX_OK;
     }
   }
@@ -1837,7 +1866,7 @@ static int solve_richardson(qphix_solver *s, void *const *x, const void *const *
   st->rsd_sq_final = r_norm_sq / rhs_sq;
   st->n_iters = iter;
   st->mv_apps = mv_outer + mv_inner;
-  st->site_flops = fl_inner * nf;
+  st->site_flops = fl_outer + fl_inner * nf;
   st->status = QPHIX_ENOCONV;
   return QPHIX_OK;
 }
@@ -1912,7 +1941,9 @@ static int solve_cg_multi(qphix_solver *s, void *const *x, const void *const *rh
     return QPHIX_OK;
   };
   TRY(true_resid(all));
+  const unsigned long sf = 12ul * (unsigned long)M->nflav;  // per right-hand side, as solve_cg counts
   for (int r = 0; r < n; r++) {
+    st[r].site_flops += (4 + 6) * sf;
     if (cp[r] <= rsd_sq[r]) {
       st[r].n_iters = 0;
       st[r].rsd_sq_final = cp[r] / chi_sq[r];
@@ -1928,7 +1959,6 @@ static int solve_cg_multi(qphix_solver *s, void *const *x, const void *const *rh
     }
     if (!act.empty()) TRY(run_blas_multi(g, B_COPY, m, nullptr));
   }
-  std::vector<int> done;
   for (int k = 0; k <= s->max_iters && !act.empty(); ++k) {
     for (int i : act) c[i] = cp[i];
     TRY(mdagm(act, pp));
@@ -1949,11 +1979,13 @@ static int solve_cg_multi(qphix_solver *s, void *const *x, const void *const *rh
     for (size_t q = 0; q < act.size(); q++) {
       const int i = act[q];
       cp[i] = tmp[q];
+      st[i].site_flops += (4 + 12) * sf;
       if (verbose) printf("CG[%d]: iter %d: r = %e target = %e\n", i, k, cp[i], rsd_sq[i]);
       if (cp[i] < rsd_sq[i]) {
         st[i].n_iters = k;
-        done.push_back(i);
+        st[i].rsd_sq_final = cp[i];
       } else {
+        st[i].site_flops += 4 * sf;
         still.push_back(i);
       }
     }
@@ -1972,10 +2004,6 @@ static int solve_cg_multi(qphix_solver *s, void *const *x, const void *const *rh
     st[i].n_iters = s->max_iters;
     st[i].rsd_sq_final = cp[i];
     st[i].status = QPHIX_ENOCONV;
-  }
-  if (!done.empty()) {
-    TRY(true_resid(done));
-    for (int i : done) st[i].rsd_sq_final = cp[i];
   }
   return QPHIX_OK;
 }
@@ -2373,7 +2401,9 @@ extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *c
   return QPHIX_OK;
 }
 
-// minvcg.h:77-387
+// minvcg.h:77-387; site_flops as minvcg.h:139-307 counts them (47 per norm,
+// 95 per axpy + norm, 24 per shift for the x_s start, then per iteration 190
+// plus 72 + 48 per shift not yet converged; not scaled by the flavours)
 extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[], int n_shift,
                                   const double *shifts, const double *rsd_target, int isign, int verbose, int cb,
                                   int *n_iters, double *rsd_sq_final, unsigned long *site_flops,
@@ -2398,9 +2428,10 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
   for (int q = 0; q < n_shift; q++) TRY(alloc_set(g, nf, p[q], s));
   std::vector<double> zeta(n_shift), zeta_prev(n_shift), beta(n_shift), rsd_sq(n_shift);
   std::vector<char> conv(n_shift, 0);
-  unsigned long mv = 0;
+  unsigned long mv = 0, fl = 0;
   double cp;
   TRY(mf_norm2(g, nf, rhs, cp));
+  fl += 47;
   for (int q = 0; q < n_shift; q++) rsd_sq[q] = rsd_target[q] * rsd_target[q] * cp;
   for (int f = 0; f < nf; f++) {
     TRY(qphix_copy_spinor(g, r[f], rhs[f]));
@@ -2418,6 +2449,7 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
     TRY(qphix_axpy_norm2(g, b, mmp[f], r[f], &t));
     c += t;
   }
+  fl += 47 + 95 + 24ul * (unsigned long)n_shift;
   for (int q = 0; q < n_shift; q++) {
     zeta_prev[q] = 1;
     zeta[q] = 1.0 / (1.0 - shifts[q] * b);
@@ -2460,7 +2492,9 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
       if (!conv[q]) {
         as[q] = a * zeta[q] * beta[q] / (zeta_prev[q] * b);
         mask_p |= 1u << q;
+        fl += 72 + 48;
       }
+    fl += 190;
     TRY(fused(1, a, mask_p, pending_x, as.data()));
     cp = c;
     const double bp = b;
@@ -2495,7 +2529,7 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
     *n_iters = k;
   }
   if (pending_x) TRY(fused(0, 0.0, 0u, pending_x, nullptr));
-  if (site_flops) *site_flops = 0;
+  if (site_flops) *site_flops = fl;
   if (mv_apps) *mv_apps = mv;
   for (void *w : s->work) qphix_free(w);
   s->work.clear();
@@ -2847,7 +2881,14 @@ extern "C" int qphix_batch_cg(qphix_batch *b, void *const x[], const void *const
     memset(&st[i], 0, sizeof(st[i]));
     st[i].n_iters = hs[i].iters;
     st[i].rsd_sq_final = hs[i].cp;
-    st[i].mv_apps = 2 * (unsigned long)(hs[i].iters + 1);
+    // the counters of invcg.h:180-416 for a solve that ran hs.loops iterations:
+    // the initial M^+M x (counted also when x = 0 skipped it) and 2 per
+    // iteration; norm2 + x - y norm, then norm2 + r/x update per iteration and
+    // aypx on every iteration that did not converge
+    const unsigned long lp = (unsigned long)hs[i].loops;
+    const unsigned long conv_in_loop = hs[i].converged && lp > 0 ? 1 : 0;
+    st[i].mv_apps = 2 + 2 * lp;
+    st[i].site_flops = 12 * (4 + 6) + 12 * (4 + 12) * lp + 12 * 4 * (lp - conv_in_loop);
     st[i].status = hs[i].converged ? QPHIX_OK : QPHIX_ENOCONV;
   }
   return QPHIX_OK;
@@ -2886,6 +2927,8 @@ extern "C" int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[
   std::vector<double> rhs_sq(n), rn2(n), rsd_t(n), tgt(n);
   std::vector<int> outer_it(n, 0);
   std::vector<unsigned long> mv(n, 0);
+  // outer site flops as inv_richardson_multiprec.h:63,87,171,191 counts them
+  std::vector<unsigned long> fl(n, (24 + 23) + (24 + 24 + 23));
   auto mdagm = [&](int i, const void *in) -> int {
     void *a[1] = {tmp2[i]}, *c[1] = {tmp[i]};
     const void *ii[1] = {in}, *mm[1] = {tmp2[i]};
@@ -2924,6 +2967,7 @@ extern "C" int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[
     for (int i = 0; i < n; i++) {
       if (rn2[i] < rsd_t[i]) continue;
       mv[i] += ist[i].mv_apps;
+      fl[i] += ist[i].site_flops + 8 * 12;
       TRY(qphix_convert_spinor(outer[i]->g, dx[i], sqrt(rn2[i]), inner->ops[i]->g, dxi[i]));
       TRY(mdagm(i, dx[i]));
       TRY(qphix_richardson_rxupdate_norm_r(outer[i]->g, x[i], r[i], dx[i], tmp[i], &rn2[i]));
@@ -2936,6 +2980,7 @@ extern "C" int qphix_batch_richardson(qphix_batch *inner, qphix_op *const outer[
     st[i].n_iters = outer_it[i];
     st[i].rsd_sq_final = rn2[i] / rhs_sq[i];
     st[i].mv_apps = mv[i];
+    st[i].site_flops = fl[i] + (24 + 24 + 23);
     st[i].status = rn2[i] < rsd_t[i] ? QPHIX_OK : QPHIX_ENOCONV;
   }
   return QPHIX_OK;

@@ -130,6 +130,7 @@ __global__ void k_cg_update(CgState *st, const double *cp, double *beta, int *ac
   if (s >= n || !active[s]) return;
   CgState &c = st[s];
   c.cp = cp[s];
+  c.loops++;
   if (c.cp < c.rsd_sq) {
     c.converged = 1;
     active[s] = 0;
@@ -154,6 +155,7 @@ __global__ void k_cg_init(CgState *st, const double *cp, const double *rsd_sq, i
   c.c = cp[s];
   c.rsd_sq = rsd_sq[s];
   c.iters = 0;
+  c.loops = 0;
   c.max_iters = max_iters;
   c.converged = c.cp <= c.rsd_sq ? 1 : 0;
   c.a = c.b = c.d = 0;

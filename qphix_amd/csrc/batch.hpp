@@ -16,10 +16,11 @@ struct SegVec {
 
 enum BatchOp { BB_NORM2 = 0, BB_COPY, BB_ZERO, BB_XMYNORM2, BB_RMAMMP, BB_AYPX };
 
-// per-system CG state in device memory
+// per-system CG state in device memory; iters is the 0-based iteration of
+// convergence (invcg.h:343), loops the iterations performed
 struct CgState {
   double c, d, a, b, cp, rsd_sq;
-  int iters, max_iters, converged, pad;
+  int iters, max_iters, converged, loops;
 };
 
 // off[nseg] = total chunks of 256 sites; out[sys] receives reductions

@@ -91,6 +91,9 @@ int main()
     double n;
     xmyNorm2Spinor(r, b, t, n, geom, 1);
     check("bicgstab_true_resid", std::sqrt(n / bnorm), 1.5e-10);
+    // the reference's counters (invbicgstab.h:158-240): it = iterations + 1
+    check("bicgstab_mv_apps", std::fabs((double)mv - (2.0 * it - 1)), 0);
+    check("bicgstab_site_flops", std::fabs((double)fl - (72.0 + 960.0 * (it - 1))), 0);
   }
   // CG through a user-defined operator (host-driven loop)
   {
@@ -106,6 +109,17 @@ int main()
     double n;
     xmyNorm2Spinor(r, b, t, n, geom, 1);
     check("generic_cg_true_resid", std::sqrt(n / bnorm), 1.5e-10);
+    // invcg.h:180-416: converged in 0-based iteration it
+    check("generic_cg_mv_apps", std::fabs((double)mv - (2.0 * it + 4)), 0);
+    check("generic_cg_site_flops", std::fabs((double)fl - (312.0 + 240.0 * it)), 0);
+    // the native CG on the same operator counts the same way
+    InvCG<double, 8, 4, false> ncg(M, 2000);
+    int it2;
+    unsigned long fl2, mv2;
+    zeroSpinor(x, geom, 1);
+    ncg(x, b, 1e-10, it2, rsd, fl2, mv2, 1, false);
+    check("native_cg_mv_apps", std::fabs((double)mv2 - (2.0 * it2 + 4)), 0);
+    check("native_cg_site_flops", std::fabs((double)fl2 - (312.0 + 240.0 * it2)), 0);
   }
   // mixed precision: FP64 Richardson around FP32 BiCGStab
   {

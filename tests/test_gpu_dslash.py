@@ -254,6 +254,59 @@ def test_gpu_anisotropic_clover_build_matches_oracle():
         assert rel(ai.download(), oracle.clover_invert(ch)) < 1e-12
 
 
+FLUX_CASES = [
+    ([4, 4, 4, 8], {(0, 1): (1, (1, -1, 0)), (2, 3): (-1, (0, 1, -1))}, 3),
+    ([4, 6, 4, 8], {(0, 2): (2, (1, -1, 0)), (1, 3): (1, (1, 1, -2))}, 4),
+    ([6, 4, 4, 4], {(0, 3): (-1, (1, 0, -1)), (1, 2): (1, (1, -1, 0))}, 5),
+]
+
+
+@pytest.mark.parametrize("c12", [False, True], ids=["c18", "c12"])
+@pytest.mark.parametrize("L,fluxes,rot", FLUX_CASES, ids=["01+23", "02+13", "03+12"])
+def test_gpu_clover_build_closed_form(L, fluxes, rot, c12):
+    """The device clover build on constant abelian fluxes (tests/closed_form.py)
+    against the closed form diag_mass - sum (c/2) sigma (x) F with
+    F = i sin(B T): A and A^-1 packed, the twisted-mass inverse blocks
+    (A + i mu g5)^-1 and the doublet blocks (tests/mesfield.cc:24-64,
+    tests/clover_term_qdp_w.h:156-195, 325-442)."""
+    import closed_form as cf
+    u, Fk = cf.abelian_flux(L, fluxes, rot)
+    vh = L[0] * L[1] * L[2] * L[3] // 2
+    g = q.Geometry(L, q.F64, c12)
+    ud = q.pack_gauge(g, u)
+    cr, ct, dm, mu, eps = 1.25, 0.75, 4.1, 0.3, 0.1
+    A = cf.clover_dense_closed_form(Fk, cr, ct, dm)
+    blocks = [A[:6, :6], A[6:, 6:]]
+    g5 = [1.0, -1.0]
+    inv = [np.linalg.inv(B) for B in blocks]
+    tminv = [np.linalg.inv(B + 1j * mu * s * np.eye(6)) for B, s in zip(blocks, g5)]
+    nd = [np.linalg.inv(B @ B + (mu * mu - eps * eps) * np.eye(6)) for B in blocks]
+    ndoff = cf.pack_triangular([eps * m for m in nd])
+    kw = dict(csw_t=ct, diag_mass=dm)
+    for cb in (0, 1):
+        a = q.clover_build(g, ud, cr, 0.0, cb, q.CLOV_A, **kw).download().reshape(vh, 72)
+        assert np.abs(a - cf.pack_triangular(blocks)[None]).max() < 1e-13
+        ai = q.clover_build(g, ud, cr, 0.0, cb, q.CLOV_AINV, **kw).download().reshape(vh, 72)
+        assert np.abs(ai - cf.pack_triangular(inv)[None]).max() < 1e-13
+        co = q.clover_build(g, ud, cr, 0.0, cb, q.CLOV_ND_OFF, mu, eps, **kw).download()
+        assert np.abs(co.reshape(vh, 72) - ndoff[None]).max() < 1e-13
+        # full blocks, through F x on a random spinor
+        xh = oracle.gen_spinor(L, 3, cb)
+        x = g.spinor().upload(xh, cb)
+        X = xh.view(np.complex128).reshape(vh, 12)
+        out = g.spinor()
+        for what, dense in ((q.CLOV_TMINV, tminv),
+                            (q.CLOV_ND_UP, [m @ (B - 1j * mu * s * np.eye(6))
+                                            for m, B, s in zip(nd, blocks, g5)]),
+                            (q.CLOV_ND_DN, [m @ (B + 1j * mu * s * np.eye(6))
+                                            for m, B, s in zip(nd, blocks, g5)])):
+            f = q.clover_build(g, ud, cr, 0.0, cb, what, mu, eps, **kw)
+            q.call("qphix_clover_apply", g.h, out.p, x.p, f.p, 1)
+            want = np.concatenate([X[:, :6] @ dense[0].T, X[:, 6:] @ dense[1].T], axis=1)
+            got = out.download(cb).view(np.complex128).reshape(vh, 12)
+            assert np.abs(got - want).max() < 1e-13, what
+
+
 @pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True)], ids=["f64", "f32", "f16"])
 def test_clover_apply_entry_point(prec, c12):
     """qphix_clover_apply: A x with packed Hermitian blocks and F x with full

@@ -388,6 +388,55 @@ def test_clover_plane_coefficients(orc):
     assert np.allclose(coef, want, atol=1e-12), coef
 
 
+# constant abelian fluxes through two disjoint planes, rotated by a constant
+# SU(3) matrix (tests/closed_form.py): lattice, {plane: (n, T)}, rotation seed
+FLUX_CASES = [
+    ([4, 4, 4, 8], {(0, 1): (1, (1, -1, 0)), (2, 3): (-1, (0, 1, -1))}, 3),
+    ([4, 6, 4, 8], {(0, 2): (2, (1, -1, 0)), (1, 3): (1, (1, 1, -2))}, 4),
+    ([6, 4, 4, 4], {(0, 3): (-1, (1, 0, -1)), (1, 2): (1, (1, -1, 0))}, 5),
+    ([4, 4, 4, 4], {(0, 1): (1, (1, -1, 0))}, None),
+]
+
+
+@pytest.mark.parametrize("L,fluxes,rot", FLUX_CASES)
+def test_field_strength_closed_form(orc, L, fluxes, rot):
+    """F_{mu nu} of a constant abelian flux is i sin(B T) on every site and
+    zero on the planes without flux (tests/mesfield.cc:24-64: four leaves,
+    (Q - Q^+)/8).  A reversed leaf halves it, a wrong factor scales it."""
+    import closed_form as cf
+    u, Fk = cf.abelian_flux(L, fluxes, rot)
+    V = L[0] * L[1] * L[2] * L[3]
+    f = np.empty(V * 6 * 18)
+    orc.lib().orc_field_strength((orc.C.c_int * 4)(*L), orc.ptr(u), orc.ptr(f))
+    F = cplx(f).reshape(V, 6, 3, 3)
+    for k, pl in enumerate(cf.PLANES):
+        assert np.abs(F[:, k] - Fk[pl][None]).max() < 1e-14, pl
+    assert sum(np.abs(Fk[pl]).max() > 0.1 for pl in cf.PLANES) == len(fluxes)
+
+
+@pytest.mark.parametrize("L,fluxes,rot", FLUX_CASES)
+def test_clover_closed_form(orc, L, fluxes, rot):
+    """The packed clover term of the abelian flux equals
+    diag_mass - sum_{mu<nu} (c/2) sigma_{mu nu} (x) F_{mu nu} with the closed-form
+    F, c = c_sw^T on the planes with t and c_sw^R on the others
+    (tests/clover_term_qdp_w.h:156-195, 325-442).  The closed form is block
+    diagonal in the chiral basis (spins 0,1 and 2,3)."""
+    import closed_form as cf
+    u, Fk = cf.abelian_flux(L, fluxes, rot)
+    cr, ct, dm = 1.25, 0.75, 4.1
+    clov = orc.clover_build(L, u, cr, 0.0, csw_t=ct, diag_mass=dm).reshape(-1, 72)
+    A = cf.clover_dense_closed_form(Fk, cr, ct, dm)
+    assert np.abs(A[:6, 6:]).max() < 1e-15 and np.abs(A[6:, :6]).max() < 1e-15
+    want = cf.pack_triangular([A[:6, :6], A[6:, 6:]])
+    assert np.abs(clov - want[None]).max() < 1e-14
+    if rot is not None:
+        assert np.abs(want[12:]).max() > 0.05  # off-diagonal entries are exercised
+    for i in range(0, clov.shape[0], 37):
+        for b in range(2):
+            assert np.allclose(cf.unpack_triangular(clov[i])[b], orc.clover_dense(clov[i], b),
+                               atol=1e-15)
+
+
 def test_clover_covariance_and_inverse(orc):
     L = [4, 4, 4, 4]
     rng = np.random.default_rng(2)
