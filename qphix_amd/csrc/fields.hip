@@ -269,247 +269,304 @@ __global__ void k_convert_words(void *out, const void *in, size_t vs, long vh) {
 
 typedef cx<double> zd;
 
-// forward link U_mu at site c from the packed arrays of its checkerboard
-template <int P, bool C12>
-__device__ inline void fwd_link(const void *const u[2], size_t vs, const Lat &l, const int c[4], int mu,
-                                zd (&U)[3][3]) {
+// ---------------------------------------------------------------------------
+// clover term: field strength per plane, then per chiral block the Hermitian
+// 6x6 matrix and what is asked of it, then the store in the field precision.
+// Three launches over chunks of sites keep every work-item's arrays in
+// registers (no private memory); the FP64 intermediates live in a chunk-sized
+// scratch allocated by the host launcher.
+// ---------------------------------------------------------------------------
+
+// U_mu at site c (either checkerboard): link 2mu+1 of the array of c's own
+// checkerboard, reading only the words that hold it
+template <int P, bool C12, int MU>
+__device__ inline void site_link(const void *const u[2], size_t vs, const Lat &l, const int c[4], zd (&U)[3][3]) {
   typedef GaugeLayout<P, C12> GL;
   typedef typename PT<P>::calc T;
-  const int cb = cb_of(c);
-  const long s = cb_site(l, c);
-  T all[8 * GL::link_reals];
-  load_range<P, GL::nw, 0, GL::nw>(u[cb], vs, s, all);
+  constexpr int pw = PT<P>::per_word;
+  constexpr int first = (2 * MU + 1) * GL::link_reals;
+  constexpr int w0 = first / pw;
+  constexpr int w1 = (first + GL::link_reals + pw - 1) / pw;
+  T buf[(w1 - w0) * pw];
+  load_range<P, GL::nw, w0, w1 - w0>(u[cb_of(c)], vs, cb_site(l, c), buf);
   cx<T> Ut[3][3];
-  make_link<C12>(all + (2 * mu + 1) * GL::link_reals, Ut);
+  make_link<C12>(buf + (first - w0 * pw), Ut);
+#pragma unroll
   for (int a = 0; a < 3; a++)
+#pragma unroll
     for (int b = 0; b < 3; b++) U[a][b] = mk((double)Ut[a][b].r, (double)Ut[a][b].i);
 }
 
-__device__ inline void mm3(const zd (&A)[3][3], const zd (&B)[3][3], zd (&C)[3][3]) {
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) C[a][b] = A[a][0] * B[0][b] + A[a][1] * B[1][b] + A[a][2] * B[2][b];
+// X = U (FIRST) or X = X U, with U = U_mu(c) or its adjoint (DAG)
+template <int P, bool C12, int MU, bool DAG, bool FIRST>
+__device__ inline void leaf_mul(const void *const u[2], size_t vs, const Lat &l, const int c[4], zd (&X)[3][3]) {
+  zd U[3][3];
+  site_link<P, C12, MU>(u, vs, l, c, U);
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) X[a][b] = DAG ? conj(U[b][a]) : U[a][b];
+  } else {
+    zd Y[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        zd t = mk(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) t = t + X[a][k] * (DAG ? conj(U[b][k]) : U[k][b]);
+        Y[a][b] = t;
+      }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) X[a][b] = Y[a][b];
+  }
 }
-__device__ inline void adj3(zd (&A)[3][3]) {
-  zd T[3][3];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) T[a][b] = conj(A[b][a]);
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) A[a][b] = T[a][b];
-}
-__device__ inline void shift2(const Lat &l, const int c[4], int mu, int dm, int nu, int dn, int o[4]) {
+
+__device__ inline void shifted(const Lat &l, const int c[4], int mu, int dm, int nu, int dn, int o[4]) {
+#pragma unroll
   for (int k = 0; k < 4; k++) o[k] = c[k];
   o[mu] = (o[mu] + dm + l.L[mu]) % l.L[mu];
   o[nu] = (o[nu] + dn + l.L[nu]) % l.L[nu];
 }
 
-// leaf product A B C D of four links (each optionally daggered) added to Q
-template <int P, bool C12>
-__device__ inline void add_leaf(const void *const u[2], size_t vs, const Lat &l, const int (&s)[4][4],
-                                const int (&m)[4], const bool (&dg)[4], zd (&Q)[3][3]) {
-  zd X[3][3], Y[3][3], Z[3][3];
-  fwd_link<P, C12>(u, vs, l, s[0], m[0], X);
-  if (dg[0]) adj3(X);
-  for (int k = 1; k < 4; k++) {
-    fwd_link<P, C12>(u, vs, l, s[k], m[k], Y);
-    if (dg[k]) adj3(Y);
-    mm3(X, Y, Z);
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) X[a][b] = Z[a][b];
-  }
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) Q[a][b] = Q[a][b] + X[a][b];
-}
-
-// Hermitian positive definite 6x6 inverse by Cholesky (M = L L^+)
-__device__ inline void hpd_inverse6(const zd (&M)[6][6], zd (&Mi)[6][6]) {
-  zd Lm[6][6];
-  double dinv[6];
-  for (int i = 0; i < 6; i++)
-    for (int j = 0; j < 6; j++) Lm[i][j] = mk(0.0, 0.0);
-  for (int j = 0; j < 6; j++) {
-    double d = M[j][j].r;
-    for (int k = 0; k < j; k++) d -= Lm[j][k].r * Lm[j][k].r + Lm[j][k].i * Lm[j][k].i;
-    d = sqrt(d);
-    Lm[j][j] = mk(d, 0.0);
-    dinv[j] = 1.0 / d;
-    for (int i = j + 1; i < 6; i++) {
-      zd s = M[i][j];
-      for (int k = 0; k < j; k++) s = s - Lm[i][k] * conj(Lm[j][k]);
-      Lm[i][j] = scale(dinv[j], s);
-    }
-  }
-  // columns of L^-1 by forward substitution, then Mi = L^-+ L^-1
-  zd Li[6][6];
-  for (int c = 0; c < 6; c++) {
-    for (int i = 0; i < 6; i++) {
-      zd s = mk(i == c ? 1.0 : 0.0, 0.0);
-      for (int k = 0; k < i; k++) s = s - Lm[i][k] * Li[k][c];
-      Li[i][c] = scale(dinv[i], s);
-    }
-  }
-  for (int i = 0; i < 6; i++)
-    for (int j = 0; j < 6; j++) {
-      zd s = mk(0.0, 0.0);
-      for (int k = 0; k < 6; k++) s = s + cmulc(Li[k][i], Li[k][j]);
-      Mi[i][j] = s;
-    }
-}
-
-template <int P, bool C12, int PO>
-__global__ void k_clover_build(const void *u0, const void *u1, size_t vs, Lat l, int cb, double cr, double ct,
-                               double diag_mass, int what, double mu, double eps, void *out) {
-  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= l.vh) return;
+// coef * F_{MU NU}(x) = coef * (Q - Q^+) / 8, Q the sum of the four clover
+// leaves at x (tests/mesfield.cc:24-64), for the chunk sites s0 + j; written
+// as 18 doubles, element e at F[e * n + j]
+template <int P, bool C12, int MU, int NU>
+__global__ void __launch_bounds__(128) k_field_strength(const void *u0, const void *u1, size_t vs, Lat l, int cb,
+                                                        long s0, long n, double coef, double *F) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
   const void *const u[2] = {u0, u1};
-  int c[4];
-  cb_coords(l, cb, s, c);
-  // F_{mu nu} = (Q - Q^+)/8 times the clover coefficient (tests/mesfield.cc:24-64)
-  zd F[6][3][3];
-  int k = 0;
-  for (int m = 0; m < 3; m++)
-    for (int n = m + 1; n < 4; n++, k++) {
-      zd Q[3][3];
-      for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) Q[a][b] = mk(0.0, 0.0);
-      int x0[4], xm[4], xn[4], xmn[4], xpm_mn[4], xpn_mm[4];
-      for (int q = 0; q < 4; q++) x0[q] = c[q];
-      shift2(l, c, m, 1, n, 0, xm);       // x+mu
-      shift2(l, c, m, 0, n, 1, xn);       // x+nu
-      shift2(l, c, m, -1, n, 0, xmn);     // x-mu
-      shift2(l, c, m, -1, n, 1, xpn_mm);  // x-mu+nu
-      shift2(l, c, m, 1, n, -1, xpm_mn);  // x+mu-nu
-      int xmm[4], xmnn[4], xnn[4];
-      shift2(l, c, m, -1, n, -1, xmnn);   // x-mu-nu
-      shift2(l, c, m, 0, n, -1, xnn);     // x-nu
-      for (int q = 0; q < 4; q++) xmm[q] = xmn[q];
-      {  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-        const int S[4][4] = {{x0[0], x0[1], x0[2], x0[3]}, {xm[0], xm[1], xm[2], xm[3]},
-                             {xn[0], xn[1], xn[2], xn[3]}, {x0[0], x0[1], x0[2], x0[3]}};
-        const int M[4] = {m, n, m, n};
-        const bool D[4] = {false, false, true, true};
-        add_leaf<P, C12>(u, vs, l, S, M, D, Q);
-      }
-      {  // U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
-        const int S[4][4] = {{x0[0], x0[1], x0[2], x0[3]}, {xpn_mm[0], xpn_mm[1], xpn_mm[2], xpn_mm[3]},
-                             {xmm[0], xmm[1], xmm[2], xmm[3]}, {xmm[0], xmm[1], xmm[2], xmm[3]}};
-        const int M[4] = {n, m, n, m};
-        const bool D[4] = {false, true, true, false};
-        add_leaf<P, C12>(u, vs, l, S, M, D, Q);
-      }
-      {  // U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
-        const int S[4][4] = {{xmm[0], xmm[1], xmm[2], xmm[3]}, {xmnn[0], xmnn[1], xmnn[2], xmnn[3]},
-                             {xmnn[0], xmnn[1], xmnn[2], xmnn[3]}, {xnn[0], xnn[1], xnn[2], xnn[3]}};
-        const int M[4] = {m, n, m, n};
-        const bool D[4] = {true, true, false, false};
-        add_leaf<P, C12>(u, vs, l, S, M, D, Q);
-      }
-      {  // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) U_mu(x)^+
-        const int S[4][4] = {{xnn[0], xnn[1], xnn[2], xnn[3]}, {xnn[0], xnn[1], xnn[2], xnn[3]},
-                             {xpm_mn[0], xpm_mn[1], xpm_mn[2], xpm_mn[3]}, {x0[0], x0[1], x0[2], x0[3]}};
-        const int M[4] = {n, m, n, m};
-        const bool D[4] = {true, false, false, true};
-        add_leaf<P, C12>(u, vs, l, S, M, D, Q);
-      }
-      const double cf = (n == 3) ? ct : cr;
-      for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) F[k][a][b] = scale(0.125 * cf, Q[a][b] - conj(Q[b][a]));
-    }
-  // Chroma clover blocks (tests/clover_term_qdp_w.h:325-442), dense 6x6 per block
-  zd A[2][6][6];
-  for (int b = 0; b < 2; b++)
-    for (int i = 0; i < 6; i++)
-      for (int j = 0; j < 6; j++) A[b][i][j] = mk(i == j ? diag_mass : 0.0, 0.0);
-  for (int a = 0; a < 3; a++) {
-    const double r0 = F[5][a][a].i - F[0][a][a].i;
-    A[0][a][a].r += r0;
-    A[0][a + 3][a + 3].r -= r0;
-    const double r1 = F[5][a][a].i + F[0][a][a].i;
-    A[1][a][a].r -= r1;
-    A[1][a + 3][a + 3].r += r1;
-  }
-  for (int a = 1; a < 3; a++)
-    for (int b = 0; b < a; b++) {
-      const zd d0 = F[0][a][b] - F[5][a][b];
-      const zd c0 = mk(-d0.i, d0.r);
-      const zd d1 = F[5][a][b] + F[0][a][b];
-      const zd c1 = mk(-d1.i, d1.r);
-      A[0][a][b] = c0;
-      A[0][a + 3][b + 3] = scale(-1.0, c0);
-      A[1][a][b] = c1;
-      A[1][a + 3][b + 3] = scale(-1.0, c1);
-    }
+  int x[4], xpm[4], xpn[4], xmm[4], xmmpn[4], xmmmn[4], xmn[4], xpmmn[4];
+  cb_coords(l, cb, s0 + j, x);
+  shifted(l, x, MU, 1, NU, 0, xpm);     // x + mu
+  shifted(l, x, MU, 0, NU, 1, xpn);     // x + nu
+  shifted(l, x, MU, -1, NU, 0, xmm);    // x - mu
+  shifted(l, x, MU, -1, NU, 1, xmmpn);  // x - mu + nu
+  shifted(l, x, MU, -1, NU, -1, xmmmn); // x - mu - nu
+  shifted(l, x, MU, 0, NU, -1, xmn);    // x - nu
+  shifted(l, x, MU, 1, NU, -1, xpmmn);  // x + mu - nu
+  zd Q[3][3], X[3][3];
+  // U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+  leaf_mul<P, C12, MU, false, true>(u, vs, l, x, X);
+  leaf_mul<P, C12, NU, false, false>(u, vs, l, xpm, X);
+  leaf_mul<P, C12, MU, true, false>(u, vs, l, xpn, X);
+  leaf_mul<P, C12, NU, true, false>(u, vs, l, x, X);
+#pragma unroll
   for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) Q[a][b] = X[a][b];
+  // U_nu(x) U_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
+  leaf_mul<P, C12, NU, false, true>(u, vs, l, x, X);
+  leaf_mul<P, C12, MU, true, false>(u, vs, l, xmmpn, X);
+  leaf_mul<P, C12, NU, true, false>(u, vs, l, xmm, X);
+  leaf_mul<P, C12, MU, false, false>(u, vs, l, xmm, X);
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) Q[a][b] = Q[a][b] + X[a][b];
+  // U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu)
+  leaf_mul<P, C12, MU, true, true>(u, vs, l, xmm, X);
+  leaf_mul<P, C12, NU, true, false>(u, vs, l, xmmmn, X);
+  leaf_mul<P, C12, MU, false, false>(u, vs, l, xmmmn, X);
+  leaf_mul<P, C12, NU, false, false>(u, vs, l, xmn, X);
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) Q[a][b] = Q[a][b] + X[a][b];
+  // U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) U_mu(x)^+
+  leaf_mul<P, C12, NU, true, true>(u, vs, l, xmn, X);
+  leaf_mul<P, C12, MU, false, false>(u, vs, l, xmn, X);
+  leaf_mul<P, C12, NU, false, false>(u, vs, l, xpmmn, X);
+  leaf_mul<P, C12, MU, true, false>(u, vs, l, x, X);
+  const double f = 0.125 * coef;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
     for (int b = 0; b < 3; b++) {
-      const zd Em = mk(-F[2][a][b].i, F[2][a][b].r) + F[4][a][b];
-      const zd Bm = mk(-F[3][a][b].i, F[3][a][b].r) - F[1][a][b];
-      A[0][a + 3][b] = Bm - Em;
-      A[1][a + 3][b] = Em + Bm;
+      const zd v = scale(f, (Q[a][b] + X[a][b]) - conj(Q[b][a] + X[b][a]));
+      F[(2 * (3 * a + b)) * n + j] = v.r;
+      F[(2 * (3 * a + b) + 1) * n + j] = v.i;
     }
-  // complete the Hermitian upper triangles
-  for (int b = 0; b < 2; b++)
-    for (int i = 0; i < 6; i++)
-      for (int j = i + 1; j < 6; j++) A[b][i][j] = conj(A[b][j][i]);
+}
 
-  typedef typename PT<PO>::calc TO;
-  if (what == 0 || what == 1 || what == 5) {  // packed Hermitian output
-    TO v[kCloverReals];
-    for (int b = 0; b < 2; b++) {
-      zd R[6][6];
-      if (what == 0) {
-        for (int i = 0; i < 6; i++)
-          for (int j = 0; j < 6; j++) R[i][j] = A[b][i][j];
-      } else if (what == 1) {
-        hpd_inverse6(A[b], R);
-      } else {
-        zd A2[6][6], Ni[6][6];
-        for (int i = 0; i < 6; i++)
-          for (int j = 0; j < 6; j++) {
-            zd sum = mk(0.0, 0.0);
-            for (int q = 0; q < 6; q++) sum = sum + A[b][i][q] * A[b][q][j];
-            A2[i][j] = sum;
-          }
-        for (int i = 0; i < 6; i++) A2[i][i].r += mu * mu - eps * eps;
-        hpd_inverse6(A2, Ni);
-        for (int i = 0; i < 6; i++)
-          for (int j = 0; j < 6; j++) R[i][j] = scale(eps, Ni[i][j]);
+// lower-triangle storage of a Hermitian 6x6 matrix: (i, j), i >= j
+__host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+__device__ inline zd hel(const zd (&M)[21], int i, int j) { return i >= j ? M[tri(i, j)] : conj(M[tri(j, i)]); }
+
+__device__ inline zd fplane(const double *F, long n, long j, int k, int a, int b) {
+  const double *p = F + (size_t)k * 18 * n;
+  return mk(p[(2 * (3 * a + b)) * n + j], p[(2 * (3 * a + b) + 1) * n + j]);
+}
+
+// chiral block b of the clover term in Chroma's packing
+// (tests/clover_term_qdp_w.h:325-442), index i = 3 * (spin - 2b) + colour:
+// with s = +1 (b = 0) or -1 (b = 1), G = i (F01 - s F23), H = (i F12 - F02) - s (i F03 + F13)
+//   A_b = [[diag_mass + G, H^+], [H, diag_mass - G]]
+// (F already carries the plane's clover coefficient)
+__device__ inline void clover_block(const double *F, long n, long j, int blk, double dm, zd (&A)[21]) {
+  const double s = blk == 0 ? 1.0 : -1.0;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b <= a; b++) {
+      const zd d = fplane(F, n, j, 0, a, b) - scale(s, fplane(F, n, j, 5, a, b));
+      const zd g = mk(-d.i, d.r);
+      A[tri(a, b)] = a == b ? mk(dm + g.r, 0.0) : g;
+      A[tri(a + 3, b + 3)] = a == b ? mk(dm - g.r, 0.0) : scale(-1.0, g);
+    }
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const zd f2 = fplane(F, n, j, 2, a, b), f3 = fplane(F, n, j, 3, a, b);
+      const zd em = mk(-f2.i, f2.r) + fplane(F, n, j, 4, a, b);
+      const zd bm = mk(-f3.i, f3.r) - fplane(F, n, j, 1, a, b);
+      A[tri(a + 3, b)] = bm - scale(s, em);
+    }
+}
+
+// M <- M^-1 for Hermitian positive definite M: Cholesky M = L L^+ in place,
+// Li = L^-1 by forward substitution, M^-1 = Li^+ Li
+__device__ inline void hpd_inverse(zd (&M)[21]) {
+  double dinv[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    double d = M[tri(c, c)].r;
+#pragma unroll
+    for (int k = 0; k < c; k++) d -= M[tri(c, k)].r * M[tri(c, k)].r + M[tri(c, k)].i * M[tri(c, k)].i;
+    d = sqrt(d);
+    dinv[c] = 1.0 / d;
+    M[tri(c, c)] = mk(d, 0.0);
+#pragma unroll
+    for (int i = c + 1; i < 6; i++) {
+      zd t = M[tri(i, c)];
+#pragma unroll
+      for (int k = 0; k < c; k++) t = t - M[tri(i, k)] * conj(M[tri(c, k)]);
+      M[tri(i, c)] = scale(dinv[c], t);
+    }
+  }
+  zd Li[21];
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    Li[tri(c, c)] = mk(dinv[c], 0.0);
+#pragma unroll
+    for (int i = c + 1; i < 6; i++) {
+      zd t = mk(0.0, 0.0);
+#pragma unroll
+      for (int k = c; k < i; k++) t = t + M[tri(i, k)] * Li[tri(k, c)];
+      Li[tri(i, c)] = scale(-dinv[i], t);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      zd t = mk(0.0, 0.0);
+#pragma unroll
+      for (int k = i; k < 6; k++) t = t + cmulc(Li[tri(k, i)], Li[tri(k, j)]);
+      M[tri(i, j)] = t;
+    }
+}
+
+// A <- A A + shift
+__device__ inline void square_shift(zd (&A)[21], double shift) {
+  zd S[21];
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j <= i; j++) {
+      zd t = mk(i == j ? shift : 0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < 6; q++) t = t + hel(A, i, q) * hel(A, q, j);
+      S[tri(i, j)] = t;
+    }
+#pragma unroll
+  for (int k = 0; k < 21; k++) A[k] = S[k];
+}
+
+// what: 0 A, 1 A^-1, 5 eps (A^2 + mu^2 - eps^2)^-1 (packed Hermitian, 36
+// reals per block); 2 (A + i mu g5)^-1 = (A^2 + mu^2)^-1 (A - i mu g5),
+// 3 (A^2 + mu^2 - eps^2)^-1 (A - i mu g5), 4 the same with + i mu g5 (full
+// complex blocks, 72 reals per block); results as FP64 into R[e * n + j]
+template <int WHAT>
+__global__ void __launch_bounds__(64) k_clover_block(const double *F, long n, double dm, double mu, double eps,
+                                                     double *R) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = blockIdx.y;
+  if (j >= n) return;
+  zd A[21];
+  clover_block(F, n, j, blk, dm, A);
+  if constexpr (WHAT == 0 || WHAT == 1 || WHAT == 5) {
+    if constexpr (WHAT == 1) hpd_inverse(A);
+    if constexpr (WHAT == 5) {
+      square_shift(A, mu * mu - eps * eps);
+      hpd_inverse(A);
+#pragma unroll
+      for (int k = 0; k < 21; k++) A[k] = scale(eps, A[k]);
+    }
+    double *o = R + (size_t)36 * blk * n;
+#pragma unroll
+    for (int i = 0; i < 6; i++) o[i * n + j] = A[tri(i, i)].r;
+#pragma unroll
+    for (int i = 1; i < 6; i++)
+#pragma unroll
+      for (int k = 0; k < i; k++) {
+        const int e = i * (i - 1) / 2 + k;
+        o[(6 + 2 * e) * n + j] = A[tri(i, k)].r;
+        o[(6 + 2 * e + 1) * n + j] = A[tri(i, k)].i;
       }
-      for (int i = 0; i < 6; i++) v[36 * b + i] = (TO)R[i][i].r;
-      for (int i = 1; i < 6; i++)
-        for (int j = 0; j < i; j++) {
-          const int e = i * (i - 1) / 2 + j;
-          v[36 * b + 6 + 2 * e] = (TO)R[i][j].r;
-          v[36 * b + 6 + 2 * e + 1] = (TO)R[i][j].i;
-        }
+  } else {
+    square_shift(A, WHAT == 2 ? mu * mu : mu * mu - eps * eps);
+    hpd_inverse(A);  // A now holds N = (A^2 + shift)^-1
+    zd B[21];
+    clover_block(F, n, j, blk, dm, B);  // the block again, for the factor on the right
+    const double g5 = blk == 0 ? 1.0 : -1.0;
+    const double tw = (WHAT == 4 ? 1.0 : -1.0) * g5 * mu;
+    double *o = R + (size_t)72 * blk * n;
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        zd t = mk(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < 6; q++) t = t + hel(A, i, q) * hel(B, q, k);
+        const zd nik = hel(A, i, k);
+        t = t + mk(-tw * nik.i, tw * nik.r);
+        o[(12 * i + 2 * k) * n + j] = t.r;
+        o[(12 * i + 2 * k + 1) * n + j] = t.i;
+      }
+  }
+}
+
+// FP64 site values R[e * n + j] -> field of precision PO (sites s0 + j), one
+// word at a time (as store_all, including the FP16 per-site scale)
+template <int PO, int N>
+__global__ void k_clover_store(const double *R, long n, long s0, size_t vs, void *out) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  typedef typename PT<PO>::calc T;
+  constexpr int pw = PT<PO>::per_word, nw = N / pw;
+  T inv = 1;
+  if constexpr (PT<PO>::scaled) {
+    float m = 0.f;
+    for (int e = 0; e < N; e++) m = fmaxf(m, fabsf((float)R[(size_t)e * n + j]));
+    inv = m > 0.f ? 1.0f / m : 0.f;
+    float *sc = reinterpret_cast<float *>(reinterpret_cast<char *>(out) + (size_t)nw * vs * 16);
+    sc[s0 + j] = m;
+  }
+  for (int w = 0; w < nw; w++) {
+    T v[pw];
+#pragma unroll
+    for (int k = 0; k < pw; k++) {
+      v[k] = (T)R[(size_t)(w * pw + k) * n + j];
+      if constexpr (PT<PO>::scaled) v[k] *= inv;
     }
-    store_all<PO, words_of<PO>(kCloverReals)>(out, vs, s, v);
-  } else {  // full blocks
-    TO v[kFullCloverReals];
-    for (int b = 0; b < 2; b++) {
-      const double g5 = b == 0 ? 1.0 : -1.0;
-      zd A2[6][6], Ni[6][6], Tm[6][6];
-      const double shift = (what == 2) ? mu * mu : mu * mu - eps * eps;
-      for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) {
-          zd sum = mk(0.0, 0.0);
-          for (int q = 0; q < 6; q++) sum = sum + A[b][i][q] * A[b][q][j];
-          A2[i][j] = sum;
-        }
-      for (int i = 0; i < 6; i++) A2[i][i].r += shift;
-      hpd_inverse6(A2, Ni);
-      // (A + i mu g5)^-1 = (A^2 + mu^2)^-1 (A - i mu g5); ND up: (..)^-1 (A - i mu g5); ND down: + i mu g5
-      const double sgn = (what == 4) ? 1.0 : -1.0;
-      for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) Tm[i][j] = A[b][i][j];
-      for (int i = 0; i < 6; i++) Tm[i][i].i += sgn * g5 * mu;
-      for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) {
-          zd sum = mk(0.0, 0.0);
-          for (int q = 0; q < 6; q++) sum = sum + Ni[i][q] * Tm[q][j];
-          v[72 * b + 12 * i + 2 * j] = (TO)sum.r;
-          v[72 * b + 12 * i + 2 * j + 1] = (TO)sum.i;
-        }
-    }
-    store_all<PO, words_of<PO>(kFullCloverReals)>(out, vs, s, v);
+    Word<PO>::template store<false>(out, widx(s0 + j, w, nw), v);
   }
 }
 
@@ -648,14 +705,65 @@ hipError_t fld_convert_clover(const GeomDesc &go, void *out, const GeomDesc &gi,
   return hipSuccess;
 }
 
+template <int P, bool C12, int MU, int NU>
+static void launch_fs(const GeomDesc &g, const Lat &l, const void *u0, const void *u1, int cb, long s0, long n,
+                      double coef, double *F, hipStream_t st) {
+  k_field_strength<P, C12, MU, NU><<<grid_for(n, 128), 128, 0, st>>>(u0, u1, g.vs, l, cb, s0, n, coef, F);
+}
+
+// all launches of one clover build, chunk by chunk, into the FP64 scratch
+// F (108 doubles per chunk site) and R (up to 144); stream-ordered, no
+// synchronisation
+template <int P, bool C12>
+static hipError_t clover_build_chunks(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct,
+                                      double dm, int cb, int what, double mu, double eps, void *out, double *F,
+                                      double *R, long chunk, hipStream_t st) {
+  Lat l = make_lat(g);
+  const bool full = what == 2 || what == 3 || what == 4;
+  for (long s0 = 0; s0 < g.vh; s0 += chunk) {
+    const long n = (g.vh - s0) < chunk ? (g.vh - s0) : chunk;
+    // planes 01 02 03 12 13 23; the planes with t take the temporal coefficient
+    launch_fs<P, C12, 0, 1>(g, l, u0, u1, cb, s0, n, cr, F + 0 * 18 * n, st);
+    launch_fs<P, C12, 0, 2>(g, l, u0, u1, cb, s0, n, cr, F + 1 * 18 * n, st);
+    launch_fs<P, C12, 0, 3>(g, l, u0, u1, cb, s0, n, ct, F + 2 * 18 * n, st);
+    launch_fs<P, C12, 1, 2>(g, l, u0, u1, cb, s0, n, cr, F + 3 * 18 * n, st);
+    launch_fs<P, C12, 1, 3>(g, l, u0, u1, cb, s0, n, ct, F + 4 * 18 * n, st);
+    launch_fs<P, C12, 2, 3>(g, l, u0, u1, cb, s0, n, ct, F + 5 * 18 * n, st);
+    const dim3 gb((unsigned)((n + 63) / 64), 2);
+    switch (what) {
+      case 0: k_clover_block<0><<<gb, 64, 0, st>>>(F, n, dm, mu, eps, R); break;
+      case 1: k_clover_block<1><<<gb, 64, 0, st>>>(F, n, dm, mu, eps, R); break;
+      case 2: k_clover_block<2><<<gb, 64, 0, st>>>(F, n, dm, mu, eps, R); break;
+      case 3: k_clover_block<3><<<gb, 64, 0, st>>>(F, n, dm, mu, eps, R); break;
+      case 4: k_clover_block<4><<<gb, 64, 0, st>>>(F, n, dm, mu, eps, R); break;
+      default: k_clover_block<5><<<gb, 64, 0, st>>>(F, n, dm, mu, eps, R); break;
+    }
+    // output in the same precision as the gauge field
+    if (full)
+      k_clover_store<P, kFullCloverReals><<<grid_for(n), 256, 0, st>>>(R, n, s0, g.vs, out);
+    else
+      k_clover_store<P, kCloverReals><<<grid_for(n), 256, 0, st>>>(R, n, s0, g.vs, out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <int P, bool C12>
 static hipError_t clover_build_po(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct, double dm,
                                   int cb, int what, double mu, double eps, void *out, hipStream_t st) {
-  Lat l = make_lat(g);
-  const dim3 gr = grid_for(g.vh, 64);
-  // output in the same precision as the gauge field
-  k_clover_build<P, C12, P><<<gr, 64, 0, st>>>(u0, u1, g.vs, l, cb, cr, ct, dm, what, mu, eps, out);
-  return hipGetLastError();
+  // chunks of at most 2^20 sites bound the FP64 scratch (field strength 108,
+  // result up to 144 doubles per site) to about 2 GiB
+  const long chunk = g.vh < (1l << 20) ? g.vh : (1l << 20);
+  double *F = nullptr, *R = nullptr;
+  hipError_t e = hipMalloc(&F, sizeof(double) * 108 * (size_t)chunk);
+  if (e == hipSuccess) e = hipMalloc(&R, sizeof(double) * kFullCloverReals * (size_t)chunk);
+  if (e == hipSuccess) e = clover_build_chunks<P, C12>(g, u0, u1, cr, ct, dm, cb, what, mu, eps, out, F, R, chunk, st);
+  // the scratch is freed only after the stream has drained
+  const hipError_t es = hipStreamSynchronize(st);
+  if (F) (void)hipFree(F);
+  if (R) (void)hipFree(R);
+  return e != hipSuccess ? e : es;
 }
 hipError_t fld_clover_build(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct, double dm,
                             int cb, int what, double mu, double eps, void *out, hipStream_t st) {
