@@ -137,6 +137,18 @@ enum {
 int qphix_clover_build(const qphix_geom *g, const void *u_cb0, const void *u_cb1,
                        double csw_r, double csw_t, double diag_mass, int cb, int what,
                        double mu, double eps, void *out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, loops that rebuild the clover term
+ * every molecular-dynamics step, harnesses that forbid hipMalloc): the same
+ * build and the same results as qphix_clover_build, with no hipMalloc,
+ * hipFree or synchronisation. Its FP64 scratch is one static device array of
+ * 32768 sites (63 MiB, held by each device the library runs on until the
+ * process ends); lattices of any size are built in chunks of that many sites.
+ * qphix_init must have run (QPHIX_EINVAL otherwise). Stream-ordered on the
+ * library stream; calls that overlap on one device share the scratch and race. */
+int qphix_clover_build_noalloc(const qphix_geom *g, const void *u_cb0, const void *u_cb1,
+                               double csw_r, double csw_t, double diag_mass, int cb, int what,
+                               double mu, double eps, void *out);
 
 /* ---- Dslash objects: Dslash / ClovDslash / TMDslash / TMClovDslash
  *      constructors (dslash_def.h:27-32, clover_dslash_def.h:25-31,

@@ -159,6 +159,8 @@ extern "C" int qphix_init(int device) {
   HIPCHK(hipHostMalloc(&g_hres, sizeof(double) * kResults, hipHostMallocDefault));
   HIPCHK(hipEventCreate(&g_ev0));
   HIPCHK(hipEventCreate(&g_ev1));
+  // the device address of the allocation-free clover build's static scratch
+  HIPCHK(fld_noalloc_init(device));
   g_init = true;
   return QPHIX_OK;
 }
@@ -413,6 +415,23 @@ extern "C" int qphix_clover_build(const qphix_geom *g, const void *u0, const voi
   // clover coefficients enter halved (tests/clover_term_qdp_w.h:165-182)
   HIPCHK(fld_clover_build(g->d, u0, u1, 0.5 * csw_r, 0.5 * csw_t, diag_mass, cb, what, mu, eps, out, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
+  return QPHIX_OK;
+}
+// The same build with no allocation, free or synchronisation: the FP64
+// scratch is a static device array of a fixed size (kNoAllocSites sites),
+// looked up by qphix_init, which must have run (it is the call that
+// allocates). Lattices of any size are built in chunks of that many sites.
+// Stream-ordered on the library stream; two calls that overlap on one device
+// share the scratch and race.
+extern "C" int qphix_clover_build_noalloc(const qphix_geom *g, const void *u0, const void *u1, double csw_r,
+                                          double csw_t, double diag_mass, int cb, int what, double mu, double eps,
+                                          void *out) {
+  if (!g_init) return fail(QPHIX_EINVAL, "qphix_clover_build_noalloc: call qphix_init first (it allocates)");
+  CHKPTR(g); CHKPTR(u0); CHKPTR(u1); CHKPTR(out);
+  TRY(check_cb(cb));
+  if (what < QPHIX_CLOV_A || what > QPHIX_CLOV_ND_OFF) return fail(QPHIX_EDOMAIN, "unknown clover output %d", what);
+  HIPCHK(fld_clover_build_noalloc(g->d, u0, u1, 0.5 * csw_r, 0.5 * csw_t, diag_mass, cb, what, mu, eps, out,
+                                  g_device, g_stream));
   return QPHIX_OK;
 }
 

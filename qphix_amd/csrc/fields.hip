@@ -765,6 +765,55 @@ static hipError_t clover_build_po(const GeomDesc &g, const void *u0, const void 
   if (R) (void)hipFree(R);
   return e != hipSuccess ? e : es;
 }
+// ---------------------------------------------------------------------------
+// allocation-free clover build
+//
+// `scratch` holds the FP64 field strength (108 doubles per site) and the
+// assembled blocks (up to 144 doubles per site) of kNoAllocSites sites; a
+// build of any lattice walks it in chunks of that many sites. Each device
+// holds its own copy of the array at its own address, looked up once per
+// device by fld_noalloc_init; host code passes only that address to kernels.
+// Every scratch entry a kernel reads is written earlier in the same call, so
+// nothing is cleared. Calls that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kNoAllocSites * (108 + kFullCloverReals)];
+static double *g_scratch_dev[kNoAllocMaxDevices] = {};
+
+hipError_t fld_noalloc_init(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  if (g_scratch_dev[device]) return hipSuccess;
+  double *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_dev[device] = p;
+  return hipSuccess;
+}
+
+template <int P, bool C12>
+static hipError_t clover_build_noalloc_po(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct,
+                                          double dm, int cb, int what, double mu, double eps, void *out,
+                                          double *scr, hipStream_t st) {
+  return clover_build_chunks<P, C12>(g, u0, u1, cr, ct, dm, cb, what, mu, eps, out, scr,
+                                     scr + 108 * kNoAllocSites, kNoAllocSites, st);
+}
+hipError_t fld_clover_build_noalloc(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct,
+                                    double dm, int cb, int what, double mu, double eps, void *out, int device,
+                                    hipStream_t st) {
+  if (device < 0 || device >= kNoAllocMaxDevices || !g_scratch_dev[device]) return hipErrorNotInitialized;
+  double *scr = g_scratch_dev[device];
+  if (g.c12) {
+    PREC_SWITCH(g.prec, return (clover_build_noalloc_po<PP, true>(g, u0, u1, cr, ct, dm, cb, what, mu, eps, out,
+                                                                   scr, st)));
+  } else {
+    switch (g.prec) {
+      case P64: return clover_build_noalloc_po<P64, false>(g, u0, u1, cr, ct, dm, cb, what, mu, eps, out, scr, st);
+      case P32: return clover_build_noalloc_po<P32, false>(g, u0, u1, cr, ct, dm, cb, what, mu, eps, out, scr, st);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipSuccess;
+}
+
 hipError_t fld_clover_build(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct, double dm,
                             int cb, int what, double mu, double eps, void *out, hipStream_t st) {
   if (g.c12) {

@@ -33,6 +33,15 @@ hipError_t fld_convert_clover(const GeomDesc &go, void *out, const GeomDesc &gi,
                               hipStream_t st);
 hipError_t fld_clover_build(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct, double dm,
                             int cb, int what, double mu, double eps, void *out, hipStream_t st);
+// allocation-free clover build: FP64 scratch for kNoAllocSites sites in a
+// static device array (about 63 MiB per device); fld_noalloc_init looks its
+// address up on the current device (once per device, at library set-up)
+constexpr long kNoAllocSites = 1l << 15;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t fld_noalloc_init(int device);
+hipError_t fld_clover_build_noalloc(const GeomDesc &g, const void *u0, const void *u1, double cr, double ct,
+                                    double dm, int cb, int what, double mu, double eps, void *out, int device,
+                                    hipStream_t st);
 
 // BLAS operations (one spinor per operand; see blas.hip)
 enum BlasOp {
