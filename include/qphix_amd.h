@@ -70,6 +70,10 @@ int qphix_set_hop_block(int block);
 /* mode 0: no tuning; 1: autotune on first use; 2: force (block, bz, slabs):
  * bz z-planes per tile, slabs t-slabs (bz = 0: natural site order) */
 int qphix_tune_config(int mode, int block, int bz, int slabs);
+/* mode 2 with (y, z) tiles of by rows (0: all of y) by bz planes, and lds
+ * bytes of LDS reserved per work-group (0..65536; the kernels do not use it,
+ * it caps the work-groups resident per CU) */
+int qphix_tune_traversal(int block, int bz, int by, int slabs, int lds);
 /* multi-RHS tiling: reg sources per work-item (1, 2 or 4; each link and
  * clover block is loaded once for all of them; 8 selects the matrix-core
  * kernel for FP32 fields, 8 sources per work-group, other precisions then

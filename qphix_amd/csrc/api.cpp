@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <complex>
 #include <map>
 #include <string>
@@ -692,6 +693,8 @@ struct HopCfg {
   double ms;
   int rt;   // right-hand sides per launch (multi-RHS keys)
   int reg;  // right-hand sides per work-item (register tile; 1 = one per work-item)
+  int by;   // rows per traversal tile (0: all of y)
+  int lds;  // dynamic LDS bytes per work-group capping the work-groups per CU (0: none)
 };
 static std::map<std::string, HopCfg> g_tuned;
 static int g_tune_enabled = -1;            // -1: read QPHIX_TUNE on first use
@@ -786,6 +789,7 @@ static void tune_hop(const GeomDesc &g, int kind, int isign, const HopArgs &a0, 
       best.bz = 0;
       best.slabs = 1;
     }
+    if (best.bz == 0 || (best.by > 0 && g.L[1] % best.by)) best.by = 0;
     return;
   }
   if (!tuning_on(g)) return;
@@ -839,32 +843,84 @@ static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs 
       return;
     }
   HopArgs a = a0;
-  a.bz = cfg.bz;
-  a.slabs = cfg.slabs;
-  double best = -1;
-  for (int reg : {1, 2, 4, 8}) {
-    if (reg == 8 && g.prec != P32) continue;
-    for (int rt : {1, 2, 4, 8, 16, 32}) {
-      if (rt > rmax || rt < reg || rt % reg) continue;
-      a.nrhs = rt;
-      for (int r = 0; r < rt; r++) {
-        a.psis[r] = a0.psis[r % n];
-        a.chis[r] = a0.chis[r % n];
-        a.outs[r] = outs[r].p;
-      }
-      if (rt == 1) {
+  // ms per source of the rmax sources applied in groups of rt (the last
+  // group may be smaller), as launch_hop_n applies them
+  auto time_groups = [&](int rt, int reg, int blk) {
+    auto go = [&]() {
+      for (int r0 = 0; r0 < rmax; r0 += rt) {
+        const int m = (rmax - r0) < rt ? (rmax - r0) : rt;
+        a.nrhs = m;
+        for (int r = 0; r < m; r++) {
+          a.psis[r] = a0.psis[r0 + r];
+          a.chis[r] = a0.chis[r0 + r];
+          a.outs[r] = outs[r0 + r].p;
+        }
         a.psi = a.psis[0];
         a.chi = a.chis[0];
         a.out = a.outs[0];
+        if (launch_hop_group(g, kind, isign, a, blk, m == 1 ? 1 : reg) != hipSuccess) return false;
       }
-      const double per = time_hop(g, kind, isign, a, cfg.block, 2, reg) / rt;
-      if (best < 0 || per < best) {
-        best = per;
-        cfg.rt = rt;
-        cfg.reg = reg;
-        cfg.ms = per;
-      }
+      return true;
+    };
+    if (!go()) {
+      (void)hipGetLastError();
+      return 1e30;
     }
+    if (hipEventRecord(g_ev0, g_stream) != hipSuccess) return 1e30;
+    for (int k = 0; k < 2; k++)
+      if (!go()) return 1e30;
+    if (hipEventRecord(g_ev1, g_stream) != hipSuccess || hipEventSynchronize(g_ev1) != hipSuccess) return 1e30;
+    float f = 0;
+    if (hipEventElapsedTime(&f, g_ev0, g_ev1) != hipSuccess) return 1e30;
+    return (double)f / (2.0 * rmax);
+  };
+  // first the tiling under the single-RHS traversal; the matrix-core form
+  // (reg 8) is not a candidate: it runs slower per source than the register
+  // tiles (qphix_tune_rhs(8, n) still forces it)
+  a.bz = cfg.bz;
+  a.slabs = cfg.slabs;
+  std::vector<HopCfg> tiles;
+  for (int reg : {1, 2, 4}) {
+    for (int rt : {1, 2, 4, 8, 16, 32, rmax}) {
+      if (rt > rmax || rt < reg || rt % reg) continue;
+      if (rt == rmax && (rt == 1 || rt == 2 || rt == 4 || rt == 8 || rt == 16 || rt == 32)) continue;
+      HopCfg c = cfg;
+      c.rt = rt;
+      c.reg = reg;
+      c.ms = time_groups(rt, reg, cfg.block);
+      tiles.push_back(c);
+    }
+  }
+  std::sort(tiles.begin(), tiles.end(), [](const HopCfg &x, const HopCfg &y) { return x.ms < y.ms; });
+  cfg = tiles[0];
+  // then the traversal for the three best tilings. With several right-hand
+  // sides per site range the source spinors of all of them share the L2, so
+  // smaller (y, z) tiles and fewer work-groups in flight per CU (an LDS
+  // reservation the kernel does not use) keep the neighbours of the sites in
+  // flight resident; the single-RHS traversal stays a candidate.
+  const int ny = g.L[1], nz = g.L[2], nt = g.L[3];
+  static const int kTiles[][2] = {{2, 0}, {4, 0}, {8, 0}, {2, 16}, {4, 8}, {4, 16}, {8, 8}, {8, 16}};
+  for (size_t t = 0; t < tiles.size() && t < 3; t++) {
+    for (int blk : {128, 256})
+      for (const auto &zy : kTiles)
+        for (int lds : {0, 40960, 54613}) {
+          const int bz = zy[0], by = zy[1];
+          if (nz % bz || (by > 0 && ny % by) || nt < 2) continue;
+          a.bz = bz;
+          a.by = by;
+          a.slabs = 1;
+          a.lds_cap = lds;
+          const double per = time_groups(tiles[t].rt, tiles[t].reg, blk);
+          if (per < cfg.ms) {
+            cfg = tiles[t];
+            cfg.block = blk;
+            cfg.bz = bz;
+            cfg.by = by;
+            cfg.slabs = 1;
+            cfg.lds = lds;
+            cfg.ms = per;
+          }
+        }
   }
 }
 
@@ -950,6 +1006,8 @@ static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void
   }
   a.bz = cfg.bz;
   a.slabs = cfg.slabs;
+  a.by = cfg.by;
+  a.lds_cap = cfg.lds;
   const int rt = n == 1 ? 1 : (cfg.rt > 0 ? cfg.rt : 1);
   const int reg = n == 1 ? 1 : cfg.reg;
   for (int r0 = 0; r0 < n; r0 += rt) {
@@ -992,6 +1050,16 @@ extern "C" int qphix_tune_config(int mode, int block, int bz, int slabs) {
   return QPHIX_OK;
 }
 
+// mode 2 of qphix_tune_config with (y, z) tiles of by rows and an LDS
+// reservation of lds bytes per work-group (it caps the work-groups per CU)
+extern "C" int qphix_tune_traversal(int block, int bz, int by, int slabs, int lds) {
+  if (by < 0 || lds < 0 || lds > 65536) return fail(QPHIX_EDOMAIN, "tile rows %d, LDS reservation %d", by, lds);
+  TRY(qphix_tune_config(2, block, bz, slabs));
+  g_forced.by = bz > 0 ? by : 0;
+  g_forced.lds = lds;
+  return QPHIX_OK;
+}
+
 extern "C" int qphix_tune_rhs(int reg, int rt) {
   if (reg != 0 && reg != 1 && reg != 2 && reg != 4 && reg != 8)
     return fail(QPHIX_EDOMAIN, "register tile %d (1, 2, 4, or 8 for the matrix-core kernel)", reg);
@@ -1010,9 +1078,9 @@ extern "C" const char *qphix_tune_summary(void) {
   out.clear();
   for (auto &kv : g_tuned) {
     char line[256];
-    snprintf(line, sizeof(line), "%s: block %d bz %d slabs %d rhs-tile %d reg-tile %d %.4f ms\n",
-             kv.first.c_str(), kv.second.block, kv.second.bz, kv.second.slabs, kv.second.rt, kv.second.reg,
-             kv.second.ms);
+    snprintf(line, sizeof(line), "%s: block %d bz %d by %d slabs %d rhs-tile %d reg-tile %d lds %d %.4f ms\n",
+             kv.first.c_str(), kv.second.block, kv.second.bz, kv.second.by, kv.second.slabs, kv.second.rt,
+             kv.second.reg, kv.second.lds, kv.second.ms);
     out += line;
   }
   return out.c_str();

@@ -39,15 +39,20 @@ struct HopArgs {
   double tbc_r[4], tbc_i[4];
   int tbc_mask;
   double alpha, mu, muinv, rho;
-  // traversal: 0 = natural site order; otherwise each of `slabs` t-slabs is
-  // walked z-block by z-block (bz planes), t innermost, so consecutive work-
-  // groups of one XCD reuse the source spinor planes they share
-  int bz, slabs;
+  // traversal: bz = 0 is the natural site order; otherwise the (y, z) cross
+  // section is cut into tiles of by rows (by = 0: all of y) by bz planes and
+  // each of `slabs` t-slabs is walked tile by tile, t innermost, so consecutive
+  // work-groups of one XCD reuse the source spinor slices they share
+  int bz, slabs, by;
   int dir;  // KD_DIR only
   // multi-RHS: nrhs > 1 applies the operator to psis[r] -> outs[r] (chis[r]);
   // the right-hand side is the fastest index of the work-group order, so the
   // work-groups sharing a site range (and its links and clover) run together
   int nrhs;
+  // dynamic LDS bytes requested per work-group (unused by the kernel): caps
+  // the work-groups resident per CU, so the sites in flight on one XCD, whose
+  // neighbour spinors must stay in its L2, fit it (0: no cap)
+  int lds_cap;
   const void *psis[kMaxRhs];
   void *outs[kMaxRhs];
   const void *chis[kMaxRhs];
@@ -381,7 +386,23 @@ struct SiteNbrs {
 __device__ inline SiteNbrs site_nbrs(const HopArgs &a, unsigned k) {
   SiteNbrs n;
   unsigned s = k;
-  if (a.bz > 0) {
+  if (a.bz > 0 && a.by > 0) {
+    // tile slice: all x, by rows, bz planes (x fastest, then y, then z)
+    const unsigned nxh = (unsigned)a.nxh, by = (unsigned)a.by, bz = (unsigned)a.bz;
+    const unsigned nts = (unsigned)(a.nt / a.slabs), nyb = (unsigned)a.ny / by, nzb = (unsigned)a.nz / bz;
+    const unsigned row = k / nxh, xh = k - row * nxh;
+    unsigned q = row / by;
+    const unsigned yi = row - q * by;
+    const unsigned zi = q % bz;
+    q /= bz;
+    const unsigned ti = q % nts;
+    q /= nts;
+    const unsigned yb = q % nyb;
+    q /= nyb;
+    const unsigned zb = q % nzb;
+    const unsigned slab = q / nzb;
+    s = (((slab * nts + ti) * (unsigned)a.nz + zb * bz + zi) * (unsigned)a.ny + yb * by + yi) * nxh + xh;
+  } else if (a.bz > 0) {
     const unsigned plane = (unsigned)(a.nxh * a.ny);
     const unsigned nts = (unsigned)(a.nt / a.slabs), bz = (unsigned)a.bz, nzb = (unsigned)(a.nz / a.bz);
     unsigned q = s / plane;

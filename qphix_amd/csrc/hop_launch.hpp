@@ -10,13 +10,13 @@ static hipError_t launch_hop3(int kind, const HopArgs &a, int block, hipStream_t
   const int nr = a.nrhs > 1 ? a.nrhs : 1;
   const dim3 grid((unsigned)(((a.vh + block - 1) / block) * nr));
   switch (kind) {
-    case KD_WILSON: hop_kernel<P, C12, ISIGN, KD_WILSON><<<grid, block, 0, st>>>(a); break;
-    case KA_WILSON: hop_kernel<P, C12, ISIGN, KA_WILSON><<<grid, block, 0, st>>>(a); break;
-    case KD_CLOVER: hop_kernel<P, C12, ISIGN, KD_CLOVER><<<grid, block, 0, st>>>(a); break;
-    case KA_CLOVER: hop_kernel<P, C12, ISIGN, KA_CLOVER><<<grid, block, 0, st>>>(a); break;
-    case KD_TM: hop_kernel<P, C12, ISIGN, KD_TM><<<grid, block, 0, st>>>(a); break;
-    case KA_TM: hop_kernel<P, C12, ISIGN, KA_TM><<<grid, block, 0, st>>>(a); break;
-    case KD_FULL: hop_kernel<P, C12, ISIGN, KD_FULL><<<grid, block, 0, st>>>(a); break;
+    case KD_WILSON: hop_kernel<P, C12, ISIGN, KD_WILSON><<<grid, block, a.lds_cap, st>>>(a); break;
+    case KA_WILSON: hop_kernel<P, C12, ISIGN, KA_WILSON><<<grid, block, a.lds_cap, st>>>(a); break;
+    case KD_CLOVER: hop_kernel<P, C12, ISIGN, KD_CLOVER><<<grid, block, a.lds_cap, st>>>(a); break;
+    case KA_CLOVER: hop_kernel<P, C12, ISIGN, KA_CLOVER><<<grid, block, a.lds_cap, st>>>(a); break;
+    case KD_TM: hop_kernel<P, C12, ISIGN, KD_TM><<<grid, block, a.lds_cap, st>>>(a); break;
+    case KA_TM: hop_kernel<P, C12, ISIGN, KA_TM><<<grid, block, a.lds_cap, st>>>(a); break;
+    case KD_FULL: hop_kernel<P, C12, ISIGN, KD_FULL><<<grid, block, a.lds_cap, st>>>(a); break;
     case KD_DIR:  // isign = +1 only (the reference hard-wires it)
       if constexpr (ISIGN == 1) {
         if (a.nrhs > 1) return hipErrorInvalidValue;
@@ -46,7 +46,7 @@ static hipError_t launch_tile3(int kind, const HopArgs &a, int block, hipStream_
   const dim3 grid((unsigned)(((a.vh + block - 1) / block) * (a.nrhs / R)));
   switch (kind) {
 #define TK(K) \
-  case K: hop_tile_kernel<P, C12, ISIGN, K, R><<<grid, block, 0, st>>>(a); break;
+  case K: hop_tile_kernel<P, C12, ISIGN, K, R><<<grid, block, a.lds_cap, st>>>(a); break;
     TK(KD_WILSON) TK(KA_WILSON) TK(KD_CLOVER) TK(KA_CLOVER) TK(KD_TM) TK(KA_TM) TK(KD_FULL)
 #undef TK
     default: return hipErrorInvalidValue;
