@@ -73,11 +73,23 @@ def cpu_baseline(seconds=12.0):
     per = dt / n
     threads = oracle.lib().orc_num_threads()
     return {"value": round(BYTES_SITE[("f64", False)] * vh / per / 1e9, 3), "unit": "GB/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "precision": "FP64, full gauge links",
+            "lattice": L, "host": "this run's host cores",
             "sample": f"oracle FP64 clover Dslash (hop + A^-1) on {threads} OpenMP threads, "
                       f"16^3x32, {vh} sites/parity, "
                       f"{n} applications in {dt:.1f} s ({per * 1e3:.1f} ms each, "
-                      f"{FLOPS_SITE_CLOVER * vh / per / 1e9:.2f} GFLOP/s)"}
+                      f"{FLOPS_SITE_CLOVER * vh / per / 1e9:.2f} GFLOP/s)",
+            # the reference's own vectorised CPU build cannot run here (its
+            # sources stay off this machine); its figure for the headline
+            # workload, measured when the reference was surveyed
+            "reference_build": {
+                "value": round(BYTES_SITE[("f32", True)] * LATT[0] * LATT[1] * LATT[2] * LATT[3] // 2
+                               / 91.4e-3 / 1e9, 1),
+                "unit": "GB/s", "cores": 8, "kind": "reference",
+                "precision": "FP32, two-row gauge links (AVX-512, VECLEN 16, SOALEN 8)",
+                "lattice": LATT, "ms_per_dslash": 91.4, "gflops": 106,
+                "host": "8-core Xeon @ 2.70 GHz with AVX-512 (the survey container), not this run's host",
+                "sample": "clover Dslash A^-1 D psi, one parity, isign +1, median of 3 repeats"}}
 
 
 def main():
@@ -195,7 +207,8 @@ def main():
     q.sync()
 
     extra["streaming_ceiling"] = ceiling
-    extra.update({"tuned": q.tune_summary().strip().splitlines(),
+    extra.update({"wall_ms_per_step": round(wall * 1e3 / args.steps, 5),
+                  "tuned": q.tune_summary().strip().splitlines(),
                   "ms_per_dslash_c12": round(per_ms, 5), "ms_per_dslash_c18": round(ms18, 5),
                   "gbs_c18": round(gbs18, 1),
                   "gflops_c12": round(FLOPS_SITE_CLOVER * vh / (per_ms * 1e-3) / 1e9, 1)})
@@ -203,6 +216,7 @@ def main():
     if not args.no_solve:
         extra["solve"] = solve_compare(q, g64, u64)
     if not args.no_block:
+        extra["clover_32x32x32x64"] = clover_32(q, max(20, args.steps))
         extra["multi_rhs_32x32x32x64"] = block_compare(q)
     if not args.no_batch:
         extra["batch_tmclover"] = batch_compare(q)
@@ -215,7 +229,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "ms_per_step": round(per_ms, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -228,7 +242,10 @@ def main():
                    "bytes_per_site": BYTES_SITE[("f32", True)]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles()},
+                     "traffic": traffic_from_profiles(),
+                     "traffic_source": "profiles/clover_dslash_f32c12_48x48x48x96_pmc.md (rocprofv3 "
+                                       "FETCH_SIZE + WRITE_SIZE, copy-calibrated; recorded with "
+                                       "--hop-config 256,6,1, not measured in this run)"},
         "extra": extra,
     }
     if not args.no_cpu:
@@ -483,6 +500,73 @@ def block_compare(q):
     out["multishift_cg"] = {"shifts": len(shifts), "seconds": round(tm, 3), "iters": nit,
                             "mv_apps": mv, "worst_true_rel_resid_over_shifts": worst}
     log(f"multishift cg 32^3x64, 12 shifts: {tm:.2f} s ({nit} iterations), worst true resid {worst:.2e}")
+    return out
+
+
+def clover_32(q, reps):
+    """The 32^3x64 clover configuration (kappa = 0.1240, c_sw = 1, FP32 fields,
+    FP64 reductions, t antiperiodic): the single-parity clover Dslash
+    A^-1 D psi in FP32 with two-row and with full gauge links against the HBM
+    roofline, and BiCGStab on the even-odd clover operator (odd target
+    checkerboard, A on odd sites, A^-1 on even sites) to 1e-7, its true
+    residual recomputed with the FP64 operator of the same seeded fields."""
+    import ctypes as C
+    L = [32, 32, 32, 64]
+    mass = 1.0 / (2 * 0.1240) - 4.0
+    g64 = q.Geometry(L, q.F64, False)
+    u64 = q.gen_gauge(g64, 2, 0.4)
+    A = q.clover_build(g64, u64, CSW, mass, 1, q.CLOV_A)
+    Ai = q.clover_build(g64, u64, CSW, mass, 0, q.CLOV_AINV)
+    M64 = q.Operator(g64, q.OP_CLOVER, u64, clov=[None, A], invclov=[Ai, None], t_boundary=-1.0)
+    out = {"lattice": L, "kappa": 0.1240, "c_sw": CSW, "sites_per_parity": g64.vh}
+    timer = q.Timer()
+    for c12 in (True, False):
+        g = q.Geometry(L, q.F32, c12)
+        u = q.convert_gauge(g, g64, u64)
+        inv = q.convert_clover(g, g64, Ai)
+        psi, o = q.gen_spinor(g, 9, 1), g.spinor()
+        D = q.Dslash(g, t_boundary=-1.0)
+        for _ in range(5):
+            D.clov_dslash(o, psi, u[0], inv, 1, 0)
+        timer.start()
+        for _ in range(reps):
+            D.clov_dslash(o, psi, u[0], inv, 1, 0)
+        ms = timer.stop() / reps
+        nb = BYTES_SITE[("f32", c12)]
+        gbs = nb * g.vh / (ms * 1e-3) / 1e9
+        out["dslash_f32_c12" if c12 else "dslash_f32_c18"] = {
+            "ms": round(ms, 5), "bytes_per_site": nb, "gbs_algorithmic": round(gbs, 1),
+            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "gflops": round(FLOPS_SITE_CLOVER * g.vh / (ms * 1e-3) / 1e9, 1)}
+        log(f"32^3x64 clover dslash f32 {'c12' if c12 else 'c18'}: {ms:.4f} ms, {gbs:.1f} GB/s "
+            f"({gbs / HBM_PEAK_GBS:.1%} of peak)")
+        if not c12:
+            continue
+        A32 = q.convert_clover(g, g64, A)
+        M32 = q.Operator(g, q.OP_CLOVER, u, clov=[None, A32], invclov=[inv, None], t_boundary=-1.0)
+        b64 = q.gen_spinor(g64, 8, 1)
+        b32 = g.spinor()
+        q.call("qphix_convert_spinor", g.h, b32.p, C.c_double(1.0), g64.h, b64.p)
+        w = g.spinor()
+        M32(w, b32, 1, 1)  # untimed: tunes the hop kernels of the operator
+        x = g.spinor()
+        s = q.InvBiCGStab(M32, 5000)
+        q.sync()
+        t0 = time.perf_counter()
+        st = s(x, b32, 1e-7)
+        q.sync()
+        ts = time.perf_counter() - t0
+        x64, r = g64.spinor(), g64.spinor()
+        q.call("qphix_convert_spinor", g64.h, x64.p, C.c_double(1.0), g.h, x.p)
+        M64(r, x64, 1, 1)
+        rn, bn = C.c_double(), C.c_double()
+        q.call("qphix_xmy_norm2", g64.h, r.p, b64.p, r.p, C.byref(rn))
+        q.call("qphix_norm2_spinor", g64.h, b64.p, C.byref(bn))
+        tr = float((rn.value / bn.value) ** 0.5)
+        out["bicgstab_f32"] = {"seconds": round(ts, 4), "iters": st.n_iters, "mv_apps": st.mv_apps,
+                               "site_flops": st.site_flops, "status": st.status, "rsd_target": 1e-7,
+                               "true_rel_resid_fp64": tr}
+        log(f"32^3x64 bicgstab f32: {ts:.3f} s, {st.n_iters} iterations, true residual (FP64) {tr:.2e}")
     return out
 
 

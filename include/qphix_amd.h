@@ -345,6 +345,14 @@ int qphix_block_cg_create(qphix_op *M, int max_iters, qphix_solver **s);
 int qphix_solve_multi(qphix_solver *s, void *const x[], const void *const rhs[], int n,
                       double rsd_target, int isign, int verbose, int target_cb, int resid_type,
                       qphix_solve_stats *st);
+/* How qphix_minvcg_solve runs its iterations (no reference counterpart; the
+ * reference keeps the recurrences on the host, minvcg.h:279-384). mode 0: FP64
+ * scalars and the zeta / beta recurrences on the host, one host round trip per
+ * iteration; 1: all of them in device memory, updated by device kernels
+ * between the vector sweeps, no host round trip; 2 (the default): as 1, with
+ * one iteration captured in a HIP graph and replayed. All three give
+ * bit-identical solutions and counters. */
+int qphix_minvcg_set_mode(qphix_solver *s, int mode);
 /* x[shift * nflav + f]; rsd_sq_final[n_shift] */
 int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[],
                        int n_shift, const double *shifts, const double *rsd_target,

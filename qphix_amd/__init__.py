@@ -490,11 +490,17 @@ def InvRichardsonMultiPrec(M_outer, inner, delta, max_iters, mdagm=False):
 
 
 class MInvCG:
-    def __init__(self, M, max_iters, max_shifts):
+    """mode: 0 host scalars, 1 FP64 scalars in device memory (eager
+    launches), 2 the same with each iteration replayed from a HIP graph
+    (the library's default)."""
+
+    def __init__(self, M, max_iters, max_shifts, mode=None):
         self.M = M
         h = VP()
         call("qphix_minvcg_create", M.h, max_iters, max_shifts, C.byref(h))
         self.h = h
+        if mode is not None:
+            call("qphix_minvcg_set_mode", h, mode)
 
     def __del__(self):
         try:

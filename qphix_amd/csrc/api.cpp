@@ -80,6 +80,10 @@ struct qphix_solver {
   // solve): the initial residual is then the right-hand side itself and the
   // operator is not applied to the zero vector
   int x_zero = 0;
+  // MInvCG control: 0 host scalars (one host round trip per iteration), 1
+  // FP64 scalars in device memory with eager launches, 2 the same with each
+  // iteration replayed from a HIP graph
+  int ms_mode = 2;
 };
 
 // ---------------------------------------------------------------------------
@@ -2488,6 +2492,133 @@ extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *c
   return QPHIX_OK;
 }
 
+// The multishift iteration with its scalars in device memory (MsState): per
+// iteration the kernels a / as_s (k_ms_pre), the vector sweep, M^+M p0, |M p0|^2,
+// b (k_ms_mid), r += b M^+M p0 with |r|^2, then the zeta / beta recurrences and
+// the convergence test (k_ms_post); no host round trip. Mode 2 captures one
+// iteration in a HIP graph and replays it. The host enqueues iteration k + 1
+// before it reads iteration k's completion flag, so the GPU never waits for
+// the host; the iteration after the last one only applies the pending x_s
+// updates (the state is frozen once done). Returns 0, -1 (not converged) or
+// a status.
+struct MsGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~MsGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+static int *g_ms_flag = nullptr;  // pinned host, 2 ints
+
+static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, const double *shifts, int isign, int cb,
+                              const std::vector<void *> &r, const std::vector<void *> &p0,
+                              std::vector<void *> &mp, std::vector<void *> &mmp,
+                              const std::vector<std::vector<void *>> &p, const std::vector<double> &zeta,
+                              const std::vector<double> &zeta_prev, const std::vector<double> &beta,
+                              const std::vector<double> &rsd_sq, double b, double c, double cp, unsigned long fl,
+                              double *rsd_sq_final, int *n_iters, unsigned long *site_flops, unsigned long *mv_apps,
+                              int *k_out) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  const int nf = M->nflav;
+  if (!g_ms_flag) HIPCHK(hipHostMalloc(&g_ms_flag, 2 * sizeof(int), hipHostMallocDefault));
+  MsState h;
+  memset(&h, 0, sizeof(h));
+  h.b = b;
+  h.c = c;
+  h.cp = cp;
+  h.nshift = n_shift;
+  h.nf = nf;
+  h.max_iters = s->max_iters;
+  h.done = s->max_iters <= 1;
+  h.site_flops = fl;
+  for (int q = 0; q < n_shift; q++) {
+    h.shifts[q] = shifts[q];
+    h.zeta[q] = zeta[q];
+    h.zeta_prev[q] = zeta_prev[q];
+    h.beta[q] = beta[q];
+    h.rsd_sq[q] = rsd_sq[q];
+  }
+  DevBuf dms;
+  if (hipMalloc(&dms.p, sizeof(MsState)) != hipSuccess) return fail(QPHIX_ENOMEM, "multishift state allocation");
+  MsState *ms = (MsState *)dms.p;
+  HIPCHK(hipMemcpyAsync(ms, &h, sizeof(h), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  auto iteration = [&]() -> int {
+    HIPCHK(ms_pre_launch(ms, g_stream));
+    for (int f = 0; f < nf; f++) {
+      MShiftArgs m;
+      memset(&m, 0, sizeof(m));
+      m.r = r[f];
+      m.p0 = p0[f];
+      m.nshift = n_shift;
+      for (int q = 0; q < n_shift; q++) {
+        m.p[q] = p[q][f];
+        m.x[q] = x[q * nf + f];
+      }
+      HIPCHK(mshift_dev_launch(g->d, m, ms, g_stream));
+    }
+    TRY(op_apply_v(M, mp, (const void *const *)p0.data(), isign, cb));
+    TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+    for (int f = 0; f < nf; f++) {
+      BlasArgs a = bargs();
+      a.x = mp[f];
+      TRY(run_blas_dev(g, B_NORM2, a, &ms->cf[f]));
+    }
+    HIPCHK(ms_mid_launch(ms, g_stream));
+    for (int f = 0; f < nf; f++)
+      HIPCHK(axpy_norm2_dev_launch(g->d, &ms->b, mmp[f], r[f], g_partials, &ms->cf[f], g_stream));
+    HIPCHK(ms_post_launch(ms, g_stream));
+    return QPHIX_OK;
+  };
+  MsGraph gr;
+  HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
+  if (s->ms_mode == 2) {
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = iteration();
+    const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
+    if (rc) return rc;
+    HIPCHK(e);
+    HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+  }
+  for (int k = 0;; k++) {
+    if (s->ms_mode == 2) {
+      HIPCHK(hipGraphLaunch(gr.exec, g_stream));
+    } else {
+      TRY(iteration());
+    }
+    HIPCHK(hipMemcpyAsync(&g_ms_flag[k & 1], &ms->done, sizeof(int), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipEventRecord(gr.ev[k & 1], g_stream));
+    if (k > 0) {
+      HIPCHK(hipEventSynchronize(gr.ev[(k - 1) & 1]));
+      if (g_ms_flag[(k - 1) & 1]) break;
+    }
+    if (k > s->max_iters + 1) break;  // the state is done after max_iters - 1 iterations
+  }
+  HIPCHK(hipMemcpyAsync(&h, ms, sizeof(h), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  *n_iters = h.iters;
+  for (int q = 0; q < n_shift; q++)
+    if (h.conv[q]) rsd_sq_final[q] = h.rsd_sq_final[q];
+  if (site_flops) *site_flops = h.site_flops;
+  if (mv_apps) *mv_apps = 2ul + 2ul * (unsigned long)h.iters;
+  *k_out = h.iters + 1;
+  return h.converged ? QPHIX_OK : -1;
+}
+
+extern "C" int qphix_minvcg_set_mode(qphix_solver *s, int mode) {
+  CHKPTR(s);
+  if (s->kind != S_MINVCG) return fail(QPHIX_EINVAL, "not a multishift solver");
+  if (mode < 0 || mode > 2) return fail(QPHIX_EDOMAIN, "multishift mode %d (0, 1 or 2)", mode);
+  s->ms_mode = mode;
+  return QPHIX_OK;
+}
+
 // minvcg.h:77-387; site_flops as minvcg.h:139-307 counts them (47 per norm,
 // 95 per axpy + norm, 24 per shift for the x_s start, then per iteration 190
 // plus 72 + 48 per shift not yet converged; not scaled by the flavours)
@@ -2542,6 +2673,15 @@ extern "C" int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *
     zeta[q] = 1.0 / (1.0 - shifts[q] * b);
     beta[q] = b / (1.0 - shifts[q] * b);
     for (int f = 0; f < nf; f++) TRY(qphix_axy(g, -beta[q], rhs[f], x[q * nf + f]));
+  }
+  if (s->ms_mode > 0) {
+    int kd = 0;
+    const int rc = minvcg_device_loop(s, x, n_shift, shifts, isign, cb, r, p0, mp, mmp, p, zeta, zeta_prev, beta,
+                                      rsd_sq, b, c, cp, fl, rsd_sq_final, n_iters, site_flops, mv_apps, &kd);
+    for (void *w : s->work) qphix_free(w);
+    s->work.clear();
+    if (rc < 0) return fail(QPHIX_ENOCONV, "multishift CG did not converge in %d iterations", kd);
+    return rc;
   }
   // one fused sweep per iteration updates p0 and every shift's p_s, and applies
   // the x_s update left pending by the previous iteration (minvcg.h:298-363)

@@ -389,6 +389,184 @@ hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st)
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// multishift CG with its scalars in device memory
+//
+// The scalar kernels evaluate the host loop's FP64 expressions in the same
+// order with contraction off, so each operation rounds as the host's does and
+// the solve is bit-identical to the one with host scalars.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(kBlasThreads) k_mshift_dev(const MShiftArgs m, const MsState *ms, size_t vs,
+                                                              long vh) {
+  typedef typename PT<P>::calc T;
+  constexpr int N = kSpinorReals;
+  const unsigned mask_x = ms->mask_x, mask_p = ms->mask_p;
+  const int do_p0 = ms->do_p0;
+  if (!do_p0 && !mask_x && !mask_p) return;
+  const T a = (T)ms->a;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < vh; s += stride) {
+    T r[N], v[N], w[N];
+    load_spinor<P, true>(m.r, vs, s, r);
+    if (do_p0) {
+      load_spinor<P, true>(m.p0, vs, s, v);
+#pragma unroll
+      for (int k = 0; k < N; k++) v[k] = r[k] + a * v[k];
+      store_spinor<P, true>(m.p0, vs, s, v);
+    }
+    for (int q = 0; q < m.nshift; q++) {
+      const unsigned bit = 1u << q;
+      if (!((mask_x | mask_p) & bit)) continue;
+      load_spinor<P, true>(m.p[q], vs, s, v);
+      if (mask_x & bit) {
+        load_spinor<P, true>(m.x[q], vs, s, w);
+        const T b = (T)ms->beta[q];
+#pragma unroll
+        for (int k = 0; k < N; k++) w[k] = w[k] - b * v[k];
+        store_spinor<P, true>(m.x[q], vs, s, w);
+      }
+      if (mask_p & bit) {
+        const T z = (T)ms->zeta[q], as = (T)ms->as[q];
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = z * r[k] + as * v[k];
+        store_spinor<P, true>(m.p[q], vs, s, v);
+      }
+    }
+  }
+}
+
+hipError_t mshift_dev_launch(const GeomDesc &g, const MShiftArgs &m, const MsState *st_dev, hipStream_t st) {
+  long nb = (g.vh + kBlasThreads - 1) / kBlasThreads;
+  if (nb > kMaxBlasBlocks) nb = kMaxBlasBlocks;
+  switch (g.prec) {
+    case P64: k_mshift_dev<P64><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
+    case P32: k_mshift_dev<P32><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
+    case P16: k_mshift_dev<P16><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// minvcg.h:279-297 (a, as_s; the counters as the reference counts them)
+__global__ void k_ms_pre(MsState *ms) {
+#pragma clang fp contract(off)
+  if (ms->done) {
+    ms->do_p0 = 0;
+    ms->mask_p = 0;
+    return;
+  }
+  const double a = ms->c / ms->cp, b = ms->b;
+  unsigned mask = 0;
+  for (int q = 0; q < ms->nshift; q++)
+    if (!ms->conv[q]) {
+      ms->as[q] = a * ms->zeta[q] * ms->beta[q] / (ms->zeta_prev[q] * b);
+      mask |= 1u << q;
+      ms->site_flops += 72 + 48;
+    }
+  ms->site_flops += 190;
+  ms->a = a;
+  ms->mask_p = mask;
+  ms->do_p0 = 1;
+  ms->cp = ms->c;
+  ms->bp = b;
+}
+
+// minvcg.h:316-320
+__global__ void k_ms_mid(MsState *ms) {
+#pragma clang fp contract(off)
+  if (ms->done) return;
+  double d = 0;
+  for (int f = 0; f < ms->nf; f++) d += ms->cf[f];
+  ms->d = d;
+  ms->b = -ms->cp / d;
+}
+
+// minvcg.h:322-384
+__global__ void k_ms_post(MsState *ms) {
+#pragma clang fp contract(off)
+  if (ms->done) {
+    ms->mask_x = 0;  // the pending x updates were applied by this iteration's sweep
+    return;
+  }
+  double c = 0;
+  for (int f = 0; f < ms->nf; f++) c += ms->cf[f];
+  ms->c = c;
+  const double a = ms->a, b = ms->b, bp = ms->bp;
+  unsigned pending = 0;
+  bool all = true;
+  for (int q = 0; q < ms->nshift; q++) {
+    if (!ms->conv[q]) {
+      const double z0 = ms->zeta[q], z1 = ms->zeta_prev[q];
+      ms->zeta_prev[q] = z0;
+      double z = z0 * z1 * bp;
+      z /= (b * a * (z1 - z0) + z1 * bp * (1.0 - ms->shifts[q] * b));
+      ms->zeta[q] = z;
+      ms->beta[q] = b * z / z0;
+      pending |= 1u << q;
+      const double css = c * z * z;
+      ms->conv[q] = css < ms->rsd_sq[q];
+      if (ms->conv[q]) ms->rsd_sq_final[q] = css;
+    }
+    all = all && ms->conv[q] != 0;
+  }
+  ms->mask_x = pending;
+  const int k = ++ms->iters;
+  if (all) {
+    ms->done = 1;
+    ms->converged = 1;
+  } else if (k + 1 >= ms->max_iters) {
+    ms->done = 1;
+  }
+}
+
+hipError_t ms_pre_launch(MsState *st_dev, hipStream_t st) {
+  k_ms_pre<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+hipError_t ms_mid_launch(MsState *st_dev, hipStream_t st) {
+  k_ms_mid<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+hipError_t ms_post_launch(MsState *st_dev, hipStream_t st) {
+  k_ms_post<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+
+// blas_site<P, B_AXPYNORM2> with the scalar read from device memory
+template <int P>
+__global__ void __launch_bounds__(kBlasThreads) k_axpy_norm2_dev(const double *b, const void *x, void *y, size_t vs,
+                                                                  long vh, double *partials) {
+  BlasArgs a;
+  a.x = x;
+  a.y = nullptr;
+  a.z = nullptr;
+  a.w = nullptr;
+  a.o0 = y;
+  a.o1 = a.o2 = a.o3 = nullptr;
+  a.a[0] = *b;
+  for (int k = 1; k < 6; k++) a.a[k] = 0;
+  double red[1] = {0};
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < vh; s += stride) blas_site<P, B_AXPYNORM2>(a, vs, s, red);
+  block_reduce_store<1>(red, partials);
+}
+
+hipError_t axpy_norm2_dev_launch(const GeomDesc &g, const double *b, const void *x, void *y, double *partials,
+                                 double *out, hipStream_t st) {
+  long nb = (g.vh + kBlasThreads - 1) / kBlasThreads;
+  if (nb > kMaxBlasBlocks) nb = kMaxBlasBlocks;
+  switch (g.prec) {
+    case P64: k_axpy_norm2_dev<P64><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
+    case P32: k_axpy_norm2_dev<P32><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
+    case P16: k_axpy_norm2_dev<P16><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
+    default: return hipErrorInvalidValue;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return blas_finalize(partials, (int)nb, 1, out, st);
+}
+
 template <int P, int OP>
 __global__ void __launch_bounds__(kBlasThreads) k_blas_multi(const MultiBlasArgs m, size_t vs, long vh,
                                                               double *partials) {

@@ -79,6 +79,35 @@ struct MShiftArgs {
 };
 hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st);
 
+// Multishift CG state in device memory (minvcg.h:279-384): the FP64 scalars
+// and per-shift recurrences of one solve, updated by single-thread kernels
+// between the vector sweeps, so that an iteration needs no host round trip
+// and can be captured in a HIP graph. cf[f] receives per-flavour reductions.
+struct MsState {
+  double a, b, bp, c, cp, d;
+  double cf[2];
+  double shifts[kMaxShifts], zeta[kMaxShifts], zeta_prev[kMaxShifts], beta[kMaxShifts], as[kMaxShifts];
+  double rsd_sq[kMaxShifts], rsd_sq_final[kMaxShifts];
+  int conv[kMaxShifts];
+  unsigned mask_x, mask_p;
+  int do_p0, nshift, nf, iters, max_iters, done, converged;
+  unsigned long site_flops;
+};
+// the vector sweep of mshift_launch with a, zeta, as, beta, the masks and
+// do_p0 read from the device state
+hipError_t mshift_dev_launch(const GeomDesc &g, const MShiftArgs &m, const MsState *st_dev, hipStream_t st);
+// start of an iteration: a = c / cp, as_s, mask_p, cp = c, bp = b (nothing
+// but do_p0 = 0 / mask_p = 0 once the solve is done)
+hipError_t ms_pre_launch(MsState *st_dev, hipStream_t st);
+// b = -cp / d, d the sum of cf over flavours
+hipError_t ms_mid_launch(MsState *st_dev, hipStream_t st);
+// end of an iteration: c from cf, zeta / beta recurrences, convergence
+hipError_t ms_post_launch(MsState *st_dev, hipStream_t st);
+// y += (*b) x and |y|^2, the scalar read from device memory; partials as
+// blas_launch, the reduction finalized into *out
+hipError_t axpy_norm2_dev_launch(const GeomDesc &g, const double *b, const void *x, void *y, double *partials,
+                                 double *out, hipStream_t st);
+
 // one BLAS operation on n right-hand sides in one launch, each with its own
 // operands and first scalar; reductions come back per right-hand side
 struct MultiBlasArgs {

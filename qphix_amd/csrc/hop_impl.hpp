@@ -108,6 +108,12 @@ template <bool C12, typename T> __device__ inline void make_link(const T *v, cx<
   }
 }
 
+// link products and clover blocks as FMA chains (cmul_f / cfma): FP16
+// storage, where the kernels are close to issue-bound; FP32 and FP64 keep
+// separate products and sums (FMA chains cut the FP32 VALU count and left the
+// memory-bound FP32 single- and multi-RHS times unchanged)
+template <int P> constexpr bool fma_of() { return PT<P>::scaled; }
+
 template <int P, bool C12> struct GaugeLayout {
   static constexpr int link_reals = C12 ? 12 : 18;
   static constexpr int nw = words_of<P>(gauge_reals(C12));
@@ -120,7 +126,7 @@ template <int P, bool C12> struct GaugeLayout {
 // the dimension has none: a branch here would split the kernel into basic
 // blocks and stop the scheduler from issuing the next hop's loads early.
 // apply_hop_h takes the projected half spinor h = (1 -/+ isign g_mu) psi.
-// FMA: the link products as FMA chains (cmul_f / cfma), used for FP16 storage
+// FMA: the link products as FMA chains (cmul_f / cfma; FP16 storage)
 template <int MU, int S, bool FWD, bool C12, bool FMA = false, typename T>
 __device__ inline void apply_hop_h(const cx<T> (&U)[3][3], const cx<T> (&h)[2][3], T coef, T tr, T ti,
                                    cx<T> (&acc)[4][3]) {
@@ -201,24 +207,24 @@ __device__ inline void hop_dim(const HopArgs &a, const void *psi_p, long s, long
     load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
-    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tr, ti, acc);
+    apply_hop<MU, ISIGN, false, C12, fma_of<P>()>(U, psi, cb_coef, tr, ti, acc);
     // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
     load_spinor<P>(psi_p, a.vs, sf, sp);
     load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
-    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr, ti, acc);
+    apply_hop<MU, -ISIGN, true, C12, fma_of<P>()>(U, psi, cf_coef, tr, ti, acc);
   } else {
     T lv[2 * GL::link_reals];
     load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
     load_spinor<P>(psi_p, a.vs, sb, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv, U);
-    apply_hop<MU, ISIGN, false, C12>(U, psi, cb_coef, tr, ti, acc);
+    apply_hop<MU, ISIGN, false, C12, fma_of<P>()>(U, psi, cb_coef, tr, ti, acc);
     load_spinor<P>(psi_p, a.vs, sf, sp);
     to_spinor(sp, psi);
     make_link<C12>(lv + GL::link_reals, U);
-    apply_hop<MU, -ISIGN, true, C12>(U, psi, cf_coef, tr, ti, acc);
+    apply_hop<MU, -ISIGN, true, C12, fma_of<P>()>(U, psi, cf_coef, tr, ti, acc);
   }
 }
 
@@ -267,7 +273,7 @@ __device__ inline void hop_half_shfl(const HopArgs &a, const void *psi_p, const 
     load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true>(a.u, a.vs, s, lv);
     make_link<C12>(lv + (FWD ? GL::link_reals : 0), U);
   }
-  apply_hop_h<MU, S, FWD, C12>(U, h, coef, tr, ti, acc);
+  apply_hop_h<MU, S, FWD, C12, fma_of<P>()>(U, h, coef, tr, ti, acc);
 }
 
 template <int P, bool C12, int ISIGN, int MU>
@@ -284,14 +290,14 @@ __device__ inline void hop_dim_shfl(const HopArgs &a, const void *psi_p, const c
 // --- clover blocks --------------------------------------------------------
 // packed Hermitian block b (36 reals) of a 72-real clover site (SCALED =
 // false: FP16 values unscaled)
-template <int P, int B, bool SCALED = true>
+template <int P, int B, bool SCALED = true, bool NT = true>
 __device__ inline void load_clover_block(const void *c, size_t vs, long s, typename PT<P>::calc *o) {
   constexpr int nw = words_of<P>(kCloverReals);
   if constexpr ((36 % PT<P>::per_word) == 0) {
-    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word, true, SCALED>(c, vs, s, o);
+    load_range<P, nw, B * (36 / PT<P>::per_word), 36 / PT<P>::per_word, NT, SCALED>(c, vs, s, o);
   } else {
     typename PT<P>::calc all[kCloverReals];
-    load_range<P, nw, 0, nw, true, SCALED>(c, vs, s, all);
+    load_range<P, nw, 0, nw, NT, SCALED>(c, vs, s, all);
 #pragma unroll
     for (int k = 0; k < 36; k++) o[k] = all[36 * B + k];
   }
@@ -465,13 +471,16 @@ __device__ inline void hop_half_r(const cx<typename PT<P>::calc> (&U)[3][3], con
     } else {
       load_spinor<P>(psi[j], a.vs, sn, sp);
       to_spinor(sp, ps);
-      apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12>(U, ps, coef, tr, ti, acc[j]);
+      apply_hop<MU, FWD ? -ISIGN : ISIGN, FWD, C12, fma_of<P>()>(U, ps, coef, tr, ti, acc[j]);
     }
   }
 }
 
 
-template <int P, bool C12, int ISIGN, int MU, int R>
+// NT: links loaded nontemporally (false where several work-groups of one
+// site range read them, the register-tiled multi-RHS launch, so that the L2
+// keeps them for the others)
+template <int P, bool C12, int ISIGN, int MU, int R, bool NT = true>
 __device__ inline void hop_dim_r(const HopArgs &a, const void *const (&psi)[R], unsigned s, unsigned sb, unsigned sf,
                                  typename PT<P>::calc cb_coef, typename PT<P>::calc cf_coef,
                                  cx<typename PT<P>::calc> (&acc)[R][4][3]) {
@@ -484,16 +493,16 @@ __device__ inline void hop_dim_r(const HopArgs &a, const void *const (&psi)[R], 
     static_assert(!PT<P>::scaled, "FP16 links are read in pairs");
     T lv[GL::link_reals];
     // backward: U_mu(x-mu)^+ (1 + isign gamma_mu) psi(x-mu)
-    load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
+    load_range<P, GL::nw, (2 * MU) * GL::link_words, GL::link_words, NT>(a.u, a.vs, s, lv);
     make_link<C12>(lv, U);
     hop_half_r<P, C12, ISIGN, MU, false, R>(U, a, psi, sb, cb_coef, tr, ti, (T)1, acc);
     // forward: U_mu(x) (1 - isign gamma_mu) psi(x+mu)
-    load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, true>(a.u, a.vs, s, lv);
+    load_range<P, GL::nw, (2 * MU + 1) * GL::link_words, GL::link_words, NT>(a.u, a.vs, s, lv);
     make_link<C12>(lv, U);
     hop_half_r<P, C12, ISIGN, MU, true, R>(U, a, psi, sf, cf_coef, tr, ti, (T)1, acc);
   } else {
     T lv[2 * GL::link_reals];
-    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, true, !PT<P>::scaled>(a.u, a.vs, s, lv);
+    load_range<P, GL::nw, MU * GL::pair_words, GL::pair_words, NT, !PT<P>::scaled>(a.u, a.vs, s, lv);
     if constexpr (PT<P>::scaled) {
       const T su = site_scale<P>(a.u, a.vs, s, GL::nw);
       make_link<C12>(lv, U, su);
@@ -540,12 +549,12 @@ __device__ inline void hop_term_r(const HopArgs &a, const void *const (&psi)[R],
 // packed Hermitian clover block B applied to R spinors (block loaded once).
 // FP16 storage: the unscaled block is applied and its site scale multiplies
 // the six outputs instead of the 36 stored reals.
-template <int P, int B, int R, bool FMA = PT<P>::scaled, typename T>
+template <int P, int B, int R, bool FMA = fma_of<P>(), bool NT = true, typename T>
 __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s, const cx<T> (&in)[R][4][3],
                                             cx<T> (&out)[R][4][3]) {
   constexpr bool raw = PT<P>::scaled;
   T m[36];
-  load_clover_block<P, B, !raw>(clov, vs, s, m);
+  load_clover_block<P, B, !raw, NT>(clov, vs, s, m);
   const T sc = raw ? site_scale<P>(clov, vs, s, words_of<P>(kCloverReals)) : (T)1;
 #pragma unroll
   for (int j = 0; j < R; j++) {
@@ -558,21 +567,21 @@ __device__ inline void clover_block_apply_r(const void *clov, size_t vs, long s,
   }
 }
 
-template <int P, int B, int R, typename T>
+template <int P, int B, int R, bool NT = true, typename T>
 __device__ inline void full_block_apply_r(const void *fcl, size_t vs, long s, const cx<T> (&in)[R][4][3],
                                           cx<T> (&out)[R][4][3]) {
   constexpr bool raw = PT<P>::scaled;
   constexpr int nw = words_of<P>(kFullCloverReals);
   constexpr int bw = nw / 2;
   T m[72];
-  load_range<P, nw, B * bw, bw, true, !raw>(fcl, vs, s, m);
+  load_range<P, nw, B * bw, bw, NT, !raw>(fcl, vs, s, m);
   const T sc = raw ? site_scale<P>(fcl, vs, s, nw) : (T)1;
 #pragma unroll
   for (int j = 0; j < R; j++) {
     cx<T> v[6], w[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) v[i] = in[j][2 * B + i / 3][i % 3];
-    full6<raw>(m, v, w);
+    full6<fma_of<P>()>(m, v, w);
 #pragma unroll
     for (int i = 0; i < 6; i++) out[j][2 * B + i / 3][i % 3] = raw ? scale(sc, w[i]) : w[i];
   }
@@ -580,7 +589,7 @@ __device__ inline void full_block_apply_r(const void *fcl, size_t vs, long s, co
 
 // accumulator start of the kernel family: 0, or the A chi part of the
 // achimbdpsi forms, for R right-hand sides at site s
-template <int P, int ISIGN, int KIND, int R>
+template <int P, int ISIGN, int KIND, int R, bool NT = true>
 __device__ inline void hop_init_r(const HopArgs &a, unsigned s, const void *const (&chi)[R],
                                   cx<typename PT<P>::calc> (&acc)[R][4][3]) {
   typedef typename PT<P>::calc T;
@@ -601,8 +610,8 @@ __device__ inline void hop_init_r(const HopArgs &a, unsigned s, const void *cons
 #pragma unroll
           for (int c = 0; c < 3; c++) acc[j][sp][c] = scale(al, ch[j][sp][c]);
     } else if constexpr (KIND == KA_CLOVER) {
-      clover_block_apply_r<P, 0, R, false>(a.clov, a.vs, s, ch, acc);
-      clover_block_apply_r<P, 1, R, false>(a.clov, a.vs, s, ch, acc);
+      clover_block_apply_r<P, 0, R, false, NT>(a.clov, a.vs, s, ch, acc);
+      clover_block_apply_r<P, 1, R, false, NT>(a.clov, a.vs, s, ch, acc);
 #pragma unroll
       for (int j = 0; j < R; j++) add_ig5((T)(ISIGN * a.rho), ch[j], acc[j]);
     } else {
@@ -632,18 +641,18 @@ __device__ inline void hop_init_r(const HopArgs &a, unsigned s, const void *cons
 
 // epilogue of the kernel family (A^-1, twisted-mass factors, full block) and
 // the store, for R right-hand sides at site s
-template <int P, int ISIGN, int KIND, int R>
+template <int P, int ISIGN, int KIND, int R, bool NT = true>
 __device__ inline void hop_finish_r(const HopArgs &a, unsigned s, const cx<typename PT<P>::calc> (&acc)[R][4][3],
                                     void *const (&out)[R]) {
   typedef typename PT<P>::calc T;
   if constexpr (KIND == KD_CLOVER || KIND == KD_FULL) {
     cx<T> o[R][4][3];
     if constexpr (KIND == KD_CLOVER) {
-      clover_block_apply_r<P, 0, R>(a.clov, a.vs, s, acc, o);
-      clover_block_apply_r<P, 1, R>(a.clov, a.vs, s, acc, o);
+      clover_block_apply_r<P, 0, R, fma_of<P>(), NT>(a.clov, a.vs, s, acc, o);
+      clover_block_apply_r<P, 1, R, fma_of<P>(), NT>(a.clov, a.vs, s, acc, o);
     } else {
-      full_block_apply_r<P, 0, R>(a.clov, a.vs, s, acc, o);
-      full_block_apply_r<P, 1, R>(a.clov, a.vs, s, acc, o);
+      full_block_apply_r<P, 0, R, NT>(a.clov, a.vs, s, acc, o);
+      full_block_apply_r<P, 1, R, NT>(a.clov, a.vs, s, acc, o);
     }
 #pragma unroll
     for (int j = 0; j < R; j++) {
@@ -687,7 +696,7 @@ constexpr bool hop_shfl(int p, int kind) { return p == P32 && kind != KD_DIR; }
 
 // wave_full: all 64 lanes of the wavefront run this site loop (the x / y
 // neighbours may then come from other lanes, hop_dim_shfl)
-template <int P, bool C12, int ISIGN, int KIND, int R>
+template <int P, bool C12, int ISIGN, int KIND, int R, bool NT = true>
 __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *const (&psi)[R], void *const (&out)[R],
                                   const void *const (&chi)[R], bool wave_full = false) {
   typedef typename PT<P>::calc T;
@@ -698,7 +707,7 @@ __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *cons
   const T ctf = (T)(n.t_hi ? a.ctf : a.ct);
 
   cx<T> acc[R][4][3];
-  hop_init_r<P, ISIGN, KIND, R>(a, s, chi, acc);
+  hop_init_r<P, ISIGN, KIND, R, NT>(a, s, chi, acc);
   if constexpr (KIND == KD_DIR) {
     switch (a.dir) {
       case 0: hop_term_r<P, C12, ISIGN, 0, true, R>(a, psi, s, n.xf, cs, acc); break;
@@ -727,12 +736,12 @@ __device__ inline void hop_body_r(const HopArgs &a, unsigned k, const void *cons
     hop_dim<P, C12, ISIGN, 2>(a, psi[0], s, n.zb, n.zf, cs, cs, acc[0]);
     hop_dim<P, C12, ISIGN, 3>(a, psi[0], s, n.tb, n.tf, ctb, ctf, acc[0]);
   } else {
-    hop_dim_r<P, C12, ISIGN, 0, R>(a, psi, s, n.xb, n.xf, cs, cs, acc);
-    hop_dim_r<P, C12, ISIGN, 1, R>(a, psi, s, n.yb, n.yf, cs, cs, acc);
-    hop_dim_r<P, C12, ISIGN, 2, R>(a, psi, s, n.zb, n.zf, cs, cs, acc);
-    hop_dim_r<P, C12, ISIGN, 3, R>(a, psi, s, n.tb, n.tf, ctb, ctf, acc);
+    hop_dim_r<P, C12, ISIGN, 0, R, NT>(a, psi, s, n.xb, n.xf, cs, cs, acc);
+    hop_dim_r<P, C12, ISIGN, 1, R, NT>(a, psi, s, n.yb, n.yf, cs, cs, acc);
+    hop_dim_r<P, C12, ISIGN, 2, R, NT>(a, psi, s, n.zb, n.zf, cs, cs, acc);
+    hop_dim_r<P, C12, ISIGN, 3, R, NT>(a, psi, s, n.tb, n.tf, ctb, ctf, acc);
   }
-  hop_finish_r<P, ISIGN, KIND, R>(a, s, acc, out);
+  hop_finish_r<P, ISIGN, KIND, R, NT>(a, s, acc, out);
 }
 
 // the hop for traversal index k (< a.vh) of one right-hand side
@@ -801,7 +810,7 @@ __global__ void __launch_bounds__(256) hop_tile_kernel(const HopArgs a) {
     out[j] = ka->outs[g * R + j];
     chi[j] = ka->chis[g * R + j];
   }
-  hop_body_r<P, C12, ISIGN, KIND, R>(a, k, psi, out, chi);
+  hop_body_r<P, C12, ISIGN, KIND, R, false>(a, k, psi, out, chi);
 }
 
 // Batched hop: one launch covers several systems (possibly on different
