@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--rhs-tile", default=None,
                     help="reg,rt: fixed multi-RHS tiling (register tile, sources per launch) instead of "
                          "autotuning (profiler runs)")
+    ap.add_argument("--rhs-traversal", default=None,
+                    help="block,bz,by,lds: fixed traversal of the multi-RHS launches (profiler runs)")
     ap.add_argument("--hop-config", default=None,
                     help="block,bz,slabs: fixed hop-kernel launch configuration instead of "
                          "autotuning (used for profiler runs so every launch is identical)")
@@ -192,8 +194,13 @@ def main():
     nr = 12
     extra = {}
     if not args.no_multi:
+        if args.rhs_traversal:
+            blk, bz, by, lds = (int(v) for v in args.rhs_traversal.split(","))
+            q.call("qphix_tune_traversal", blk, bz, by, 1, lds)
         extra["multi_rhs_12"] = multi_rhs(q, D, g32, u32, inv32, nr, max(1, args.steps // 10),
                                           per_ms, timer)
+        if args.rhs_traversal:
+            q.call("qphix_tune_config", 1, 256, 0, 1)
 
     if not args.no_family:
         extra["kernels_48x48x48x96"] = kernel_family(q, g64, u64, g32, u32, psi, out, vh,
@@ -270,7 +277,10 @@ def multi_rhs(q, D, g32, u32, inv32, nr, reps, per_ms, timer):
     gbs = bytes_multi / (ms_multi * 1e-3) / 1e9
     log(f"clover dslash f32 c12, {nr} rhs: {ms_multi:.3f} ms ({ms_multi / nr:.4f} ms per rhs), "
         f"{gbs:.1f} GB/s algorithmic")
-    out = {"tuned": [ln for ln in q.tune_summary().splitlines() if f" n{nr}" in ln],
+    nb = 2  # the multi-RHS tuning class of nr sources
+    while nb * 2 <= nr and nb < 32:
+        nb *= 2
+    out = {"tuned": [ln for ln in q.tune_summary().splitlines() if f" n{nb}:" in ln],
            "ms": round(ms_multi, 4), "ms_per_rhs": round(ms_multi / nr, 5),
            "gbs_algorithmic": round(gbs, 1),
            "speedup_per_rhs_vs_single": round(per_ms / (ms_multi / nr), 3)}
@@ -458,7 +468,8 @@ def block_compare(q):
 
     out = {"lattice": L, "n_rhs": n, "tol": tol}
     w = [g32.spinor() for _ in range(n)]
-    M32.apply_multi(w, rhs32, 1, 1)  # untimed: tunes the multi-RHS kernels
+    M32.apply_multi(w, rhs32, 1, 1)  # untimed: tunes the multi-RHS kernels (both signs)
+    M32.apply_multi(w, rhs32, -1, 1)
     q.sync()
     xb = [g32.spinor() for _ in range(n)]
     t0 = time.perf_counter()
@@ -477,16 +488,21 @@ def block_compare(q):
     log(f"block cg 32^3x64 x{n}: {tb:.2f} s ({stb[0].n_iters} block iterations), lockstep cg "
         f"{tl:.2f} s (max {out['lockstep_cg']['max_iters']} iterations)")
     del xb, xl, w, rhs32
-    # multishift CG, one Gaussian source, FP64
+    # multishift CG, one Gaussian source, FP64: scalars and recurrences on
+    # the host (one round trip per iteration), then in device memory with the
+    # iteration replayed from a HIP graph (the library's default)
     shifts = [10 ** (-4 + 4 * k / 11) for k in range(12)]
     b = rhs64[12]
-    xs = [g64.spinor() for _ in shifts]
-    ms = q.MInvCG(M64, 5000, len(shifts))
-    q.sync()
-    t0 = time.perf_counter()
-    nit, fin, mv = ms(xs, b, shifts, [1e-8] * len(shifts))
-    q.sync()
-    tm = time.perf_counter() - t0
+    per_mode = {}
+    for mode in (0, 1, 2):
+        xs = [g64.spinor() for _ in shifts]
+        ms = q.MInvCG(M64, 5000, len(shifts), mode=mode)
+        q.sync()
+        t0 = time.perf_counter()
+        nit, fin, mv = ms(xs, b, shifts, [1e-8] * len(shifts))
+        q.sync()
+        per_mode[mode] = (time.perf_counter() - t0, nit)
+    tm = per_mode[2][0]
     t1, t2 = g64.spinor(), g64.spinor()
     worst = 0.0
     for sg, x in zip(shifts, xs):
@@ -497,9 +513,13 @@ def block_compare(q):
         q.call("qphix_xmy_norm2", g64.h, t2.p, b.p, t2.p, C.byref(rn))
         q.call("qphix_norm2_spinor", g64.h, b.p, C.byref(bn))
         worst = max(worst, (rn.value / bn.value) ** 0.5)
-    out["multishift_cg"] = {"shifts": len(shifts), "seconds": round(tm, 3), "iters": nit,
-                            "mv_apps": mv, "worst_true_rel_resid_over_shifts": worst}
-    log(f"multishift cg 32^3x64, 12 shifts: {tm:.2f} s ({nit} iterations), worst true resid {worst:.2e}")
+    out["multishift_cg"] = {"shifts": len(shifts), "seconds": round(tm, 4), "iters": nit,
+                            "mv_apps": mv, "worst_true_rel_resid_over_shifts": worst,
+                            "seconds_host_scalars": round(per_mode[0][0], 4),
+                            "seconds_device_scalars_eager": round(per_mode[1][0], 4),
+                            "seconds_device_scalars_graph": round(per_mode[2][0], 4)}
+    log(f"multishift cg 32^3x64, 12 shifts: {tm:.3f} s ({nit} iterations; host scalars "
+        f"{per_mode[0][0]:.3f} s, device scalars eager {per_mode[1][0]:.3f} s), worst true resid {worst:.2e}")
     return out
 
 

@@ -897,17 +897,17 @@ static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs 
   }
   std::sort(tiles.begin(), tiles.end(), [](const HopCfg &x, const HopCfg &y) { return x.ms < y.ms; });
   cfg = tiles[0];
-  // then the traversal for the three best tilings. With several right-hand
-  // sides per site range the source spinors of all of them share the L2, so
+  // then the traversal for the best tiling. With several right-hand sides
+  // per site range the source spinors of all of them share the L2, so
   // smaller (y, z) tiles and fewer work-groups in flight per CU (an LDS
   // reservation the kernel does not use) keep the neighbours of the sites in
   // flight resident; the single-RHS traversal stays a candidate.
   const int ny = g.L[1], nz = g.L[2], nt = g.L[3];
-  static const int kTiles[][2] = {{2, 0}, {4, 0}, {8, 0}, {2, 16}, {4, 8}, {4, 16}, {8, 8}, {8, 16}};
-  for (size_t t = 0; t < tiles.size() && t < 3; t++) {
+  static const int kTiles[][2] = {{4, 0}, {8, 0}, {2, 16}, {4, 8}, {6, 16}, {8, 16}};
+  for (size_t t = 0; t < tiles.size() && t < 1; t++) {
     for (int blk : {128, 256})
       for (const auto &zy : kTiles)
-        for (int lds : {0, 40960, 54613}) {
+        for (int lds : {0, 40960}) {
           const int bz = zy[0], by = zy[1];
           if (nz % bz || (by > 0 && ny % by) || nt < 2) continue;
           a.bz = bz;
@@ -993,8 +993,12 @@ static int launch_hop_n(const qphix_dslash *d, int kind, int isign, int cb, void
   }
   HopCfg cfg = it->second;
   if (n > 1) {
+    // one tuning per power-of-two class of n: solvers that drop converged
+    // right-hand sides apply the operator to a shrinking set
+    int nb = 2;
+    while (nb * 2 <= n && nb < kMaxRhs) nb *= 2;
     char mkey[160];
-    snprintf(mkey, sizeof(mkey), "%s n%d", key, n);
+    snprintf(mkey, sizeof(mkey), "%s n%d", key, nb);
     auto mt = g_tuned.find(mkey);
     if (mt == g_tuned.end()) {
       HopCfg c = cfg;
@@ -2497,10 +2501,12 @@ extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *c
 // b (k_ms_mid), r += b M^+M p0 with |r|^2, then the zeta / beta recurrences and
 // the convergence test (k_ms_post); no host round trip. Mode 2 captures one
 // iteration in a HIP graph and replays it. The host enqueues iteration k + 1
-// before it reads iteration k's completion flag, so the GPU never waits for
-// the host; the iteration after the last one only applies the pending x_s
-// updates (the state is frozen once done). Returns 0, -1 (not converged) or
-// a status.
+// before it reads iteration k's completion flag, so the GPU does not wait for
+// the host, except when the residual trend (MsState::progress of the last two
+// iterations read) says iteration k may be the last: then it waits for k
+// first, so that no full iteration is spent after the solve is done. Once done
+// the state is frozen; a closing sweep applies the x_s updates left pending.
+// Returns 0, -1 (not converged) or a status.
 struct MsGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2512,7 +2518,7 @@ struct MsGraph {
       if (e) (void)hipEventDestroy(e);
   }
 };
-static int *g_ms_flag = nullptr;  // pinned host, 2 ints
+static MsState::HostView *g_ms_view = nullptr;  // pinned host, 2 entries
 
 static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, const double *shifts, int isign, int cb,
                               const std::vector<void *> &r, const std::vector<void *> &p0,
@@ -2525,7 +2531,7 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
   qphix_op *M = s->M;
   const qphix_geom *g = M->g;
   const int nf = M->nflav;
-  if (!g_ms_flag) HIPCHK(hipHostMalloc(&g_ms_flag, 2 * sizeof(int), hipHostMallocDefault));
+  if (!g_ms_view) HIPCHK(hipHostMalloc(&g_ms_view, 2 * sizeof(MsState::HostView), hipHostMallocCoherent));
   MsState h;
   memset(&h, 0, sizeof(h));
   h.b = b;
@@ -2536,6 +2542,7 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
   h.max_iters = s->max_iters;
   h.done = s->max_iters <= 1;
   h.site_flops = fl;
+  h.hview = g_ms_view;
   for (int q = 0; q < n_shift; q++) {
     h.shifts[q] = shifts[q];
     h.zeta[q] = zeta[q];
@@ -2548,8 +2555,7 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
   MsState *ms = (MsState *)dms.p;
   HIPCHK(hipMemcpyAsync(ms, &h, sizeof(h), hipMemcpyHostToDevice, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
-  auto iteration = [&]() -> int {
-    HIPCHK(ms_pre_launch(ms, g_stream));
+  auto sweep = [&]() -> int {
     for (int f = 0; f < nf; f++) {
       MShiftArgs m;
       memset(&m, 0, sizeof(m));
@@ -2562,6 +2568,10 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
       }
       HIPCHK(mshift_dev_launch(g->d, m, ms, g_stream));
     }
+    return QPHIX_OK;
+  };
+  auto iteration = [&]() -> int {
+    TRY(sweep());
     TRY(op_apply_v(M, mp, (const void *const *)p0.data(), isign, cb));
     TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
     for (int f = 0; f < nf; f++) {
@@ -2575,6 +2585,7 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
     HIPCHK(ms_post_launch(ms, g_stream));
     return QPHIX_OK;
   };
+  HIPCHK(ms_pre_launch(ms, g_stream));  // the first iteration's a, as_s
   MsGraph gr;
   HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
@@ -2586,20 +2597,31 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
     HIPCHK(e);
     HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
   }
+  double prev = -1, prev2 = -1;  // progress of the last two iterations read
   for (int k = 0;; k++) {
     if (s->ms_mode == 2) {
       HIPCHK(hipGraphLaunch(gr.exec, g_stream));
     } else {
       TRY(iteration());
     }
-    HIPCHK(hipMemcpyAsync(&g_ms_flag[k & 1], &ms->done, sizeof(int), hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipEventRecord(gr.ev[k & 1], g_stream));
     if (k > 0) {
       HIPCHK(hipEventSynchronize(gr.ev[(k - 1) & 1]));
-      if (g_ms_flag[(k - 1) & 1]) break;
+      if (g_ms_view[(k - 1) & 1].done) break;
+      prev2 = prev;
+      prev = g_ms_view[(k - 1) & 1].progress;
+    }
+    // iteration k may end the solve (geometric extrapolation of the residual,
+    // or the iteration limit): wait for it before enqueueing another one
+    const bool last_near = (prev > 0 && prev2 > 0 && prev * (prev / prev2) < 1.5) || k + 2 >= s->max_iters;
+    if (last_near) {
+      HIPCHK(hipEventSynchronize(gr.ev[k & 1]));
+      if (g_ms_view[k & 1].done) break;
     }
     if (k > s->max_iters + 1) break;  // the state is done after max_iters - 1 iterations
   }
+  TRY(sweep());  // the pending x_s updates (a no-op if the last iteration applied them)
+  HIPCHK(ms_post_launch(ms, g_stream));
   HIPCHK(hipMemcpyAsync(&h, ms, sizeof(h), hipMemcpyDeviceToHost, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
   *n_iters = h.iters;

@@ -449,13 +449,8 @@ hipError_t mshift_dev_launch(const GeomDesc &g, const MShiftArgs &m, const MsSta
 }
 
 // minvcg.h:279-297 (a, as_s; the counters as the reference counts them)
-__global__ void k_ms_pre(MsState *ms) {
+__device__ inline void ms_prepare(MsState *ms) {
 #pragma clang fp contract(off)
-  if (ms->done) {
-    ms->do_p0 = 0;
-    ms->mask_p = 0;
-    return;
-  }
   const double a = ms->c / ms->cp, b = ms->b;
   unsigned mask = 0;
   for (int q = 0; q < ms->nshift; q++)
@@ -472,6 +467,15 @@ __global__ void k_ms_pre(MsState *ms) {
   ms->bp = b;
 }
 
+__global__ void k_ms_pre(MsState *ms) {
+  if (ms->done) {
+    ms->do_p0 = 0;
+    ms->mask_p = 0;
+    return;
+  }
+  ms_prepare(ms);
+}
+
 // minvcg.h:316-320
 __global__ void k_ms_mid(MsState *ms) {
 #pragma clang fp contract(off)
@@ -482,11 +486,23 @@ __global__ void k_ms_mid(MsState *ms) {
   ms->b = -ms->cp / d;
 }
 
+__device__ inline void ms_publish(MsState *ms) {
+  MsState::HostView *v = ms->hview + (ms->posts & 1);
+  v->done = ms->done;
+  v->progress = ms->progress;
+  __threadfence_system();
+  ms->posts++;
+}
+
 // minvcg.h:322-384
 __global__ void k_ms_post(MsState *ms) {
 #pragma clang fp contract(off)
   if (ms->done) {
-    ms->mask_x = 0;  // the pending x updates were applied by this iteration's sweep
+    // the x_s updates left pending were applied by this iteration's sweep
+    ms->mask_x = 0;
+    ms->mask_p = 0;
+    ms->do_p0 = 0;
+    ms_publish(ms);
     return;
   }
   double c = 0;
@@ -495,6 +511,7 @@ __global__ void k_ms_post(MsState *ms) {
   const double a = ms->a, b = ms->b, bp = ms->bp;
   unsigned pending = 0;
   bool all = true;
+  double prog = 0;
   for (int q = 0; q < ms->nshift; q++) {
     if (!ms->conv[q]) {
       const double z0 = ms->zeta[q], z1 = ms->zeta_prev[q];
@@ -505,12 +522,15 @@ __global__ void k_ms_post(MsState *ms) {
       ms->beta[q] = b * z / z0;
       pending |= 1u << q;
       const double css = c * z * z;
+      const double ratio = css / ms->rsd_sq[q];
+      if (ratio > prog) prog = ratio;
       ms->conv[q] = css < ms->rsd_sq[q];
       if (ms->conv[q]) ms->rsd_sq_final[q] = css;
     }
     all = all && ms->conv[q] != 0;
   }
   ms->mask_x = pending;
+  ms->progress = prog;
   const int k = ++ms->iters;
   if (all) {
     ms->done = 1;
@@ -518,6 +538,13 @@ __global__ void k_ms_post(MsState *ms) {
   } else if (k + 1 >= ms->max_iters) {
     ms->done = 1;
   }
+  if (ms->done) {
+    ms->mask_p = 0;
+    ms->do_p0 = 0;
+  } else {
+    ms_prepare(ms);
+  }
+  ms_publish(ms);
 }
 
 hipError_t ms_pre_launch(MsState *st_dev, hipStream_t st) {

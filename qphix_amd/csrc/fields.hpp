@@ -92,16 +92,28 @@ struct MsState {
   unsigned mask_x, mask_p;
   int do_p0, nshift, nf, iters, max_iters, done, converged;
   unsigned long site_flops;
+  // largest c zeta_s^2 / target_s over the shifts tested by the last
+  // iteration (below 1 for a shift that converged there); lets the host
+  // foresee the last iteration
+  double progress;
+  // written by every k_ms_post into hview[posts & 1] (pinned host memory):
+  // the host reads the state of a finished iteration without a copy command
+  struct HostView {
+    int done, pad;
+    double progress;
+  } *hview;
+  int posts;
 };
 // the vector sweep of mshift_launch with a, zeta, as, beta, the masks and
 // do_p0 read from the device state
 hipError_t mshift_dev_launch(const GeomDesc &g, const MShiftArgs &m, const MsState *st_dev, hipStream_t st);
-// start of an iteration: a = c / cp, as_s, mask_p, cp = c, bp = b (nothing
-// but do_p0 = 0 / mask_p = 0 once the solve is done)
+// start of the first iteration: a = c / cp, as_s, mask_p, cp = c, bp = b
 hipError_t ms_pre_launch(MsState *st_dev, hipStream_t st);
 // b = -cp / d, d the sum of cf over flavours
 hipError_t ms_mid_launch(MsState *st_dev, hipStream_t st);
-// end of an iteration: c from cf, zeta / beta recurrences, convergence
+// end of an iteration: c from cf, zeta / beta recurrences, convergence, then
+// the start of the next iteration as ms_pre_launch (or, once the solve is
+// done, only the pending x_s updates for the next sweep)
 hipError_t ms_post_launch(MsState *st_dev, hipStream_t st);
 // y += (*b) x and |y|^2, the scalar read from device memory; partials as
 // blas_launch, the reduction finalized into *out
