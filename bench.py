@@ -49,6 +49,40 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
+def runtime_copy_rate(nbytes=4 << 30, reps=10):
+    """The HIP runtime's own device-to-device copy (hipMemcpyAsync of a
+    4 GiB buffer), bytes read plus bytes written per second: a streaming
+    ceiling measured with code that is not this project's."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        return None
+    try:
+        hip.hipMemset(src, 1, C.c_size_t(nbytes))
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        for _ in range(2):
+            hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None)
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        per = ms.value / reps
+        gbs = 2 * nbytes / (per * 1e-3) / 1e9
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return {"bytes": nbytes, "ms": round(per, 4), "gbs_read_plus_write": round(gbs, 1),
+                "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's FP64 clover Dslash (hop + A^-1) timed on the host on a
     16^3x32 sample with its OpenMP site loops on all the threads OpenMP is
@@ -213,6 +247,11 @@ def main():
         q.copy(g32, out, psi)
     q.sync()
 
+    rc = runtime_copy_rate()
+    if rc:
+        ceiling["hip_runtime_d2d_copy"] = rc
+        log(f"hipMemcpy device to device, 4 GiB: {rc['ms']:.3f} ms, {rc['gbs_read_plus_write']:.1f} GB/s "
+            f"({rc['frac_of_peak']:.1%} of peak)")
     extra["streaming_ceiling"] = ceiling
     extra.update({"wall_ms_per_step": round(wall * 1e3 / args.steps, 5),
                   "tuned": q.tune_summary().strip().splitlines(),
