@@ -28,7 +28,8 @@ KAPPA = 0.1245
 MASS = 1.0 / (2 * KAPPA) - 4.0   # alpha = 4 + m = 1/(2 kappa)
 CSW = 1.0
 HBM_PEAK_GBS = 8000.0
-BYTES_SITE = {("f32", True): 864, ("f32", False): 1056, ("f64", False): 2112, ("f16", True): 448}
+BYTES_SITE = {("f32", True): 864, ("f32", False): 1056, ("f64", False): 2112, ("f16", True): 448,
+              ("bf16", True): 448}
 FLOPS_SITE_CLOVER = 1824
 RHS_TILE = []  # forced multi-RHS tiling (--rhs-tile), empty: autotuned
 
@@ -684,11 +685,36 @@ def solve_compare(q, g64, u64):
                                     "rel_err_vs_fp64": float((dn.value / xn.value) ** 0.5)}
     out["speedup"] = round(t64 / tm, 3)
     log(f"solve: fp64 {t64:.2f} s ({st.n_iters} it), mixed {tm:.2f} s, speedup {t64 / tm:.2f}")
+    # the same defect correction around a BF16-storage inner BiCGStab
+    gb = q.Geometry(LATT, q.BF16, True)
+    ub = q.convert_gauge(gb, g64, u64)
+    clovb = [None, q.convert_clover(gb, g64, clov64[1])]
+    invb = [q.convert_clover(gb, g64, inv64[0]), None]
+    Mb = q.Operator(gb, q.OP_CLOVER, ub, clov=clovb, invclov=invb, t_boundary=-1.0)
+    wb = gb.spinor()
+    Mb(wb, gb.spinor(), 1, 1)
+    xb = g64.spinor()
+    sb = q.InvRichardsonMultiPrec(M64, q.InvBiCGStab(Mb, 5000), 1e-2, 200)
+    q.sync()
+    t0 = time.perf_counter()
+    stb = sb(xb, b, 1e-10)
+    q.sync()
+    tb = time.perf_counter() - t0
+    M64(r, xb, 1, 1)
+    q.call("qphix_xmy_norm2", g64.h, r.p, b.p, r.p, C.byref(rn))
+    q.call("qphix_xmy_norm2", g64.h, d.p, xb.p, x64.p, C.byref(dn))
+    out["mixed_bf16_richardson"] = {"seconds": round(tb, 3), "outer_iters": stb.n_iters,
+                                    "mv_apps": stb.mv_apps, "status": stb.status,
+                                    "true_rel_resid": float((rn.value / bn.value) ** 0.5),
+                                    "rel_err_vs_fp64": float((dn.value / xn.value) ** 0.5),
+                                    "speedup": round(t64 / tb, 3)}
+    log(f"solve: mixed with bf16 inner {tb:.3f} s ({stb.n_iters} outer), speedup {t64 / tb:.2f}")
     # the clover Dslash A^-1 D psi of each precision the solve uses
     timer = q.Timer()
     per = {}
     for name, g, u, inv, nb in (("f64_c18", g64, u64, inv64[0], BYTES_SITE[("f64", False)]),
-                                ("f16_c12", g16, u16, inv16[0], BYTES_SITE[("f16", True)])):
+                                ("f16_c12", g16, u16, inv16[0], BYTES_SITE[("f16", True)]),
+                                ("bf16_c12", gb, ub, invb[0], BYTES_SITE[("bf16", True)])):
         Dp = q.Dslash(g, t_boundary=-1.0)
         ps, o = q.gen_spinor(g, 5, 1), g.spinor()
         for _ in range(5):

@@ -42,8 +42,11 @@ enum {
   QPHIX_ENODEV = 7       /* no HIP device                                      */
 };
 
-/* precision of a geometry (Geometry<FT,...> FT = double / float / half) */
-enum { QPHIX_F64 = 0, QPHIX_F32 = 1, QPHIX_F16 = 2 };
+/* precision of a geometry (Geometry<FT,...> FT = double / float / half;
+ * arith_type.h:8-17 computes half in float). F16 (IEEE half) and BF16
+ * (bfloat16) fields store a per-site FP32 scale, compute in FP32 and need
+ * two-row gauge compression. */
+enum { QPHIX_F64 = 0, QPHIX_F32 = 1, QPHIX_F16 = 2, QPHIX_BF16 = 3 };
 
 typedef struct qphix_geom qphix_geom;
 typedef struct qphix_dslash qphix_dslash;

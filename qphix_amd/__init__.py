@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqphix_amd.so")
 INCLUDE = os.path.join(os.path.dirname(_HERE), "include", "qphix_amd.h")
 
-F64, F32, F16 = 0, 1, 2
+F64, F32, F16, BF16 = 0, 1, 2, 3
 OP_WILSON, OP_CLOVER, OP_TMWILSON, OP_TMCLOVER, OP_NDTMWILSON, OP_NDTMCLOVER = range(6)
 CLOV_A, CLOV_AINV, CLOV_TMINV, CLOV_ND_UP, CLOV_ND_DN, CLOV_ND_OFF = range(6)
 RESID_ABSOLUTE, RESID_RELATIVE = 0, 1

@@ -29,6 +29,7 @@ hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int 
 extern template hipError_t launch_hop_prec<P64>(int, bool, int, const HopArgs &, int, hipStream_t);
 extern template hipError_t launch_hop_prec<P32>(int, bool, int, const HopArgs &, int, hipStream_t);
 extern template hipError_t launch_hop_prec<P16>(int, bool, int, const HopArgs &, int, hipStream_t);
+extern template hipError_t launch_hop_prec<PB16>(int, bool, int, const HopArgs &, int, hipStream_t);
 template <int P>
 hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *segs, const int *off, int nseg,
                                  const int *active, int grid, hipStream_t st);
@@ -38,6 +39,8 @@ extern template hipError_t launch_hop_batch_prec<P32>(int, bool, int, const HopS
                                                       int, hipStream_t);
 extern template hipError_t launch_hop_batch_prec<P16>(int, bool, int, const HopSeg *, const int *, int, const int *,
                                                       int, hipStream_t);
+extern template hipError_t launch_hop_batch_prec<PB16>(int, bool, int, const HopSeg *, const int *, int, const int *,
+                                                       int, hipStream_t);
 }  // namespace qphix_amd
 
 struct qphix_geom {
@@ -212,9 +215,10 @@ extern "C" int qphix_geom_create(const int latt[4], int prec, int c12, qphix_geo
   for (int k = 0; k < 4; k++)
     if (latt[k] < 2 || (latt[k] & 1))
       return fail(QPHIX_EDOMAIN, "lattice extent %d in dimension %d must be even and >= 2", latt[k], k);
-  if (prec != QPHIX_F64 && prec != QPHIX_F32 && prec != QPHIX_F16)
+  if (prec != QPHIX_F64 && prec != QPHIX_F32 && prec != QPHIX_F16 && prec != QPHIX_BF16)
     return fail(QPHIX_EDOMAIN, "unknown precision %d", prec);
-  if (prec == QPHIX_F16 && !c12) return fail(QPHIX_EDOMAIN, "F16 geometries require two-row gauge compression");
+  if ((prec == QPHIX_F16 || prec == QPHIX_BF16) && !c12)
+    return fail(QPHIX_EDOMAIN, "F16 and BF16 geometries require two-row gauge compression");
   // the kernels index sites of one parity (padded to whole 256-site blocks)
   // and their neighbours in 32 bits
   const long vh = (long)latt[0] * latt[1] * latt[2] * latt[3] / 2;
@@ -230,7 +234,8 @@ extern "C" int qphix_geom_create(const int latt[4], int prec, int c12, qphix_geo
     switch (prec) {
       case QPHIX_F64: g->bytes[k] = field_bytes<P64>(nr[k], g->d.vs); break;
       case QPHIX_F32: g->bytes[k] = field_bytes<P32>(nr[k], g->d.vs); break;
-      default: g->bytes[k] = field_bytes<P16>(nr[k], g->d.vs); break;
+      case QPHIX_F16: g->bytes[k] = field_bytes<P16>(nr[k], g->d.vs); break;
+      default: g->bytes[k] = field_bytes<PB16>(nr[k], g->d.vs); break;
     }
   }
   *out = g;
@@ -709,7 +714,8 @@ static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const H
   switch (g.prec) {
     case P64: return launch_hop_prec<P64>(kind, g.c12 != 0, isign, a, block, g_stream);
     case P32: return launch_hop_prec<P32>(kind, g.c12 != 0, isign, a, block, g_stream);
-    default: return launch_hop_prec<P16>(kind, g.c12 != 0, isign, a, block, g_stream);
+    case P16: return launch_hop_prec<P16>(kind, g.c12 != 0, isign, a, block, g_stream);
+    default: return launch_hop_prec<PB16>(kind, g.c12 != 0, isign, a, block, g_stream);
   }
 }
 
@@ -723,7 +729,8 @@ static hipError_t launch_hop_tile_raw(const GeomDesc &g, int kind, int isign, co
   switch (g.prec) {
     case P64: return launch_hop_tile_prec<P64>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
     case P32: return launch_hop_tile_prec<P32>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
-    default: return launch_hop_tile_prec<P16>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
+    case P16: return launch_hop_tile_prec<P16>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
+    default: return launch_hop_tile_prec<PB16>(kind, g.c12 != 0, isign, reg, a, block, g_stream);
   }
 }
 
@@ -765,6 +772,7 @@ static bool tuning_on(const GeomDesc &g) {
 static size_t spinor_bytes(const GeomDesc &g) {
   return g.prec == P64 ? field_bytes<P64>(kSpinorReals, g.vs)
          : g.prec == P32 ? field_bytes<P32>(kSpinorReals, g.vs) : field_bytes<P16>(kSpinorReals, g.vs);
+  // (PB16 fields have the P16 size)
 }
 
 // time `reps` launches of a (reg > 0: the multi-RHS group launch); returns ms
@@ -2240,7 +2248,7 @@ extern "C" int qphix_block_gram(const qphix_geom *g, const void *const a[], int 
                                 double *out) {
   BLAS_PRE(); CHKPTR(a); CHKPTR(b); CHKPTR(out);
   if (na < 1 || nb < 1 || na > kMaxRhs || nb > kMaxRhs) return fail(QPHIX_EDOMAIN, "block sizes %d x %d", na, nb);
-  if (g->d.prec == P16) return fail(QPHIX_EDOMAIN, "block BLAS needs FP64 or FP32 fields");
+  if (g->d.prec == P16 || g->d.prec == PB16) return fail(QPHIX_EDOMAIN, "block BLAS needs FP64 or FP32 fields");
   TRY(block_buffers());
   BlockArgs ba;
   memset(&ba, 0, sizeof(ba));
@@ -2266,7 +2274,7 @@ extern "C" int qphix_block_combine(const qphix_geom *g, const void *const a[], i
                                    const void *const y[], void *const o[], int nb) {
   BLAS_PRE(); CHKPTR(a); CHKPTR(mat); CHKPTR(o);
   if (na < 1 || nb < 1 || na > kMaxRhs || nb > kMaxRhs) return fail(QPHIX_EDOMAIN, "block sizes %d x %d", na, nb);
-  if (g->d.prec == P16) return fail(QPHIX_EDOMAIN, "block BLAS needs FP64 or FP32 fields");
+  if (g->d.prec == P16 || g->d.prec == PB16) return fail(QPHIX_EDOMAIN, "block BLAS needs FP64 or FP32 fields");
   TRY(block_buffers());
   BlockArgs ba;
   memset(&ba, 0, sizeof(ba));
@@ -2486,7 +2494,8 @@ extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *c
   if (n < 1) return fail(QPHIX_EDOMAIN, "number of right-hand sides %d", n);
   if (s->kind == S_BLOCKCG) {
     if (n > kMaxRhs) return fail(QPHIX_EDOMAIN, "block CG takes at most %d right-hand sides, got %d", kMaxRhs, n);
-    if (s->M->g->d.prec == P16) return fail(QPHIX_EDOMAIN, "block CG needs FP64 or FP32 fields");
+    if (s->M->g->d.prec == P16 || s->M->g->d.prec == PB16)
+      return fail(QPHIX_EDOMAIN, "block CG needs FP64 or FP32 fields");
     return solve_block_cg(s, x, rhs, n, rsd_target, isign, verbose, target_cb, resid_type, st);
   }
   for (int r0 = 0; r0 < n; r0 += kMaxRhs) {
@@ -2971,13 +2980,15 @@ static hipError_t batch_hop(const qphix_batch *b, int stage, int kind, int isign
     switch (b->prec) {
       case P64: return launch_hop_batch_tile_prec<P64>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
       case P32: return launch_hop_batch_tile_prec<P32>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
-      default: return launch_hop_batch_tile_prec<P16>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
+      case P16: return launch_hop_batch_tile_prec<P16>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
+      default: return launch_hop_batch_tile_prec<PB16>(kind, b->c12 != 0, isign, segs, b->d_hoff, b->nhseg, b->d_active, grid, g_stream);
     }
   }
   switch (b->prec) {
     case P64: return launch_hop_batch_prec<P64>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
     case P32: return launch_hop_batch_prec<P32>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
-    default: return launch_hop_batch_prec<P16>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
+    case P16: return launch_hop_batch_prec<P16>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
+    default: return launch_hop_batch_prec<PB16>(kind, b->c12 != 0, isign, segs, b->d_off, b->n, b->d_active, batch_grid(b), g_stream);
   }
 }
 static hipError_t batch_blas(const qphix_batch *b, int step, int op, const double *scal, double *out, bool gate) {

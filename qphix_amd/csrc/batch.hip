@@ -181,7 +181,8 @@ hipError_t batch_blas_launch(int prec, int op, const SegVec *segs, const int *of
   switch (prec) {
     case P64: e = blas_prec<P64>(op, segs, off, nseg, active, scal, partials, grid, st); break;
     case P32: e = blas_prec<P32>(op, segs, off, nseg, active, scal, partials, grid, st); break;
-    default: e = blas_prec<P16>(op, segs, off, nseg, active, scal, partials, grid, st); break;
+    case P16: e = blas_prec<P16>(op, segs, off, nseg, active, scal, partials, grid, st); break;
+    default: e = blas_prec<PB16>(op, segs, off, nseg, active, scal, partials, grid, st); break;
   }
   if (e != hipSuccess) return e;
   if (out && (op == BB_NORM2 || op == BB_RMAMMP || op == BB_XMYNORM2)) {

@@ -337,6 +337,7 @@ hipError_t blas_launch(const GeomDesc &g, int op, const BlasArgs &a, double *par
     case P64: return launch_prec<P64>(g, op, a, partials, (int)nb, st);
     case P32: return launch_prec<P32>(g, op, a, partials, (int)nb, st);
     case P16: return launch_prec<P16>(g, op, a, partials, (int)nb, st);
+    case PB16: return launch_prec<PB16>(g, op, a, partials, (int)nb, st);
   }
   return hipErrorInvalidValue;
 }
@@ -384,6 +385,7 @@ hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st)
     case P64: k_mshift<P64><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
     case P32: k_mshift<P32><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
     case P16: k_mshift<P16><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
+    case PB16: k_mshift<PB16><<<(int)nb, kBlasThreads, 0, st>>>(m, g.vs, g.vh); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -443,6 +445,7 @@ hipError_t mshift_dev_launch(const GeomDesc &g, const MShiftArgs &m, const MsSta
     case P64: k_mshift_dev<P64><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
     case P32: k_mshift_dev<P32><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
     case P16: k_mshift_dev<P16><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
+    case PB16: k_mshift_dev<PB16><<<(int)nb, kBlasThreads, 0, st>>>(m, st_dev, g.vs, g.vh); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -587,6 +590,7 @@ hipError_t axpy_norm2_dev_launch(const GeomDesc &g, const double *b, const void 
     case P64: k_axpy_norm2_dev<P64><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
     case P32: k_axpy_norm2_dev<P32><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
     case P16: k_axpy_norm2_dev<P16><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
+    case PB16: k_axpy_norm2_dev<PB16><<<(int)nb, kBlasThreads, 0, st>>>(b, x, y, g.vs, g.vh, partials); break;
     default: return hipErrorInvalidValue;
   }
   const hipError_t e = hipGetLastError();
@@ -667,7 +671,8 @@ hipError_t blas_multi_launch(const GeomDesc &g, int op, const MultiBlasArgs &m, 
   switch (g.prec) {
     case P64: e = multi_prec<P64>(g, op, m, partials, (int)nb, st); break;
     case P32: e = multi_prec<P32>(g, op, m, partials, (int)nb, st); break;
-    default: e = multi_prec<P16>(g, op, m, partials, (int)nb, st); break;
+    case P16: e = multi_prec<P16>(g, op, m, partials, (int)nb, st); break;
+    default: e = multi_prec<PB16>(g, op, m, partials, (int)nb, st); break;
   }
   if (e != hipSuccess) return e;
   const int nr = nred_of(op);

@@ -31,9 +31,12 @@
 
 namespace qphix_amd {
 
-enum Prec { P64 = 0, P32 = 1, P16 = 2 };
+// P16: IEEE half, PB16: bfloat16 (the FP32 exponent with an 8-bit
+// significand); both store a per-site FP32 scale and compute in FP32
+enum Prec { P64 = 0, P32 = 1, P16 = 2, PB16 = 3 };
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 template <int P> struct PT;
 template <> struct PT<P64> {
@@ -47,6 +50,11 @@ template <> struct PT<P32> {
   static constexpr bool scaled = false;
 };
 template <> struct PT<P16> {
+  typedef float calc;
+  static constexpr int per_word = 8;
+  static constexpr bool scaled = true;
+};
+template <> struct PT<PB16> {
   typedef float calc;
   static constexpr int per_word = 8;
   static constexpr bool scaled = true;
@@ -122,6 +130,24 @@ template <> struct Word<P16> {
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = (_Float16)in[k];
     half8_t *p = reinterpret_cast<half8_t *>(base) + i;
+    if (NT) __builtin_nontemporal_store(v, p); else *p = v;
+  }
+};
+
+// bfloat16 words: a value widens to FP32 by a shift; FP32 narrows with
+// round-to-nearest-even (v_cvt_pk_bf16_f32)
+template <> struct Word<PB16> {
+  template <bool NT> __device__ static inline void load(const void *base, size_t i, float *o) {
+    const bf8_t *p = reinterpret_cast<const bf8_t *>(base) + i;
+    const bf8_t v = NT ? __builtin_nontemporal_load(p) : *p;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = (float)v[k];
+  }
+  template <bool NT> __device__ static inline void store(void *base, size_t i, const float *in) {
+    bf8_t v;
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = (__bf16)in[k];
+    bf8_t *p = reinterpret_cast<bf8_t *>(base) + i;
     if (NT) __builtin_nontemporal_store(v, p); else *p = v;
   }
 };

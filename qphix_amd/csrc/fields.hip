@@ -587,6 +587,7 @@ static inline Lat make_lat(const GeomDesc &g) {
     case P64: { constexpr int PP = P64; F; } break; \
     case P32: { constexpr int PP = P32; F; } break; \
     case P16: { constexpr int PP = P16; F; } break; \
+    case PB16: { constexpr int PP = PB16; F; } break; \
     default: return hipErrorInvalidValue; \
   }
 
@@ -610,6 +611,7 @@ static hipError_t conv_spinor_in(int pin, void *out, const void *in, size_t vs, 
     case P64: k_convert_spinor<PO, P64><<<grid_for(vh), 256, 0, st>>>(out, in, vs, vh, scale); break;
     case P32: k_convert_spinor<PO, P32><<<grid_for(vh), 256, 0, st>>>(out, in, vs, vh, scale); break;
     case P16: k_convert_spinor<PO, P16><<<grid_for(vh), 256, 0, st>>>(out, in, vs, vh, scale); break;
+    case PB16: k_convert_spinor<PO, PB16><<<grid_for(vh), 256, 0, st>>>(out, in, vs, vh, scale); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -647,6 +649,7 @@ static hipError_t conv_gauge_in(const GeomDesc &gi, void *out, const void *in, s
       case P64: k_convert_gauge<PO, P64, CO, true><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
       case P32: k_convert_gauge<PO, P32, CO, true><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
       case P16: k_convert_gauge<PO, P16, CO, true><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
+      case PB16: k_convert_gauge<PO, PB16, CO, true><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
       default: return hipErrorInvalidValue;
     }
   } else {
@@ -691,6 +694,7 @@ static hipError_t conv_words_in(int pin, void *out, const void *in, size_t vs, l
     case P64: k_convert_words<PO, P64, N><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
     case P32: k_convert_words<PO, P32, N><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
     case P16: k_convert_words<PO, P16, N><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
+    case PB16: k_convert_words<PO, PB16, N><<<gr, 128, 0, st>>>(out, in, vs, vh); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -391,12 +391,15 @@ struct SiteNbrs {
 // sites, and 64-bit division would cost a branchy library sequence.
 __device__ inline SiteNbrs site_nbrs(const HopArgs &a, unsigned k) {
   SiteNbrs n;
-  unsigned s = k;
-  if (a.bz > 0 && a.by > 0) {
-    // tile slice: all x, by rows, bz planes (x fastest, then y, then z)
-    const unsigned nxh = (unsigned)a.nxh, by = (unsigned)a.by, bz = (unsigned)a.bz;
-    const unsigned nts = (unsigned)(a.nt / a.slabs), nyb = (unsigned)a.ny / by, nzb = (unsigned)a.nz / bz;
-    const unsigned row = k / nxh, xh = k - row * nxh;
+  const unsigned nxh = (unsigned)a.nxh, ny = (unsigned)a.ny, nz = (unsigned)a.nz, nt = (unsigned)a.nt;
+  unsigned s, xh, y, z, t;
+  if (a.bz > 0) {
+    // tile slice: all x, by rows (all of y for by = 0), bz planes (x fastest,
+    // then y, then z); then t within a slab, then the tiles, then the slabs
+    const unsigned by = a.by > 0 ? (unsigned)a.by : ny, bz = (unsigned)a.bz;
+    const unsigned nts = nt / (unsigned)a.slabs, nyb = ny / by, nzb = nz / bz;
+    const unsigned row = k / nxh;
+    xh = k - row * nxh;
     unsigned q = row / by;
     const unsigned yi = row - q * by;
     const unsigned zi = q % bz;
@@ -407,28 +410,20 @@ __device__ inline SiteNbrs site_nbrs(const HopArgs &a, unsigned k) {
     q /= nyb;
     const unsigned zb = q % nzb;
     const unsigned slab = q / nzb;
-    s = (((slab * nts + ti) * (unsigned)a.nz + zb * bz + zi) * (unsigned)a.ny + yb * by + yi) * nxh + xh;
-  } else if (a.bz > 0) {
-    const unsigned plane = (unsigned)(a.nxh * a.ny);
-    const unsigned nts = (unsigned)(a.nt / a.slabs), bz = (unsigned)a.bz, nzb = (unsigned)(a.nz / a.bz);
-    unsigned q = s / plane;
-    const unsigned p = s - q * plane;
-    const unsigned zi = q % bz;
-    q /= bz;
-    const unsigned ti = q % nts;
-    q /= nts;
-    const unsigned zb = q % nzb;
-    const unsigned slab = q / nzb;
-    s = ((slab * nts + ti) * (unsigned)a.nz + zb * bz + zi) * plane + p;
+    y = yb * by + yi;
+    z = zb * bz + zi;
+    t = slab * nts + ti;
+    s = ((t * nz + z) * ny + y) * nxh + xh;
+  } else {
+    s = k;
+    unsigned r = s;
+    xh = r % nxh;
+    r /= nxh;
+    y = r % ny;
+    r /= ny;
+    z = r % nz;
+    t = r / nz;
   }
-  const unsigned nxh = (unsigned)a.nxh, ny = (unsigned)a.ny, nz = (unsigned)a.nz, nt = (unsigned)a.nt;
-  unsigned r = s;
-  const unsigned xh = r % nxh;
-  r /= nxh;
-  const unsigned y = r % ny;
-  r /= ny;
-  const unsigned z = r % nz;
-  const unsigned t = r / nz;
   const unsigned nx = 2 * nxh;
   const unsigned x = 2 * xh + ((y + z + t + (unsigned)a.cb) & 1);
   const unsigned row = s - xh;
@@ -760,7 +755,7 @@ __device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, voi
 #ifndef QPHIX_HOP_WAVES16
 #define QPHIX_HOP_WAVES16 4
 #endif
-constexpr int hop_min_waves(int p) { return p == P16 ? QPHIX_HOP_WAVES16 : 1; }
+constexpr int hop_min_waves(int p) { return (p == P16 || p == PB16) ? QPHIX_HOP_WAVES16 : 1; }
 
 template <int P, bool C12, int ISIGN, int KIND>
 __global__ void __launch_bounds__(256, hop_min_waves(P)) hop_kernel(const HopArgs a) {
@@ -831,7 +826,7 @@ struct HopSeg {
 // one by one (the register-tiled form ran 63% slower). Two waves leave them
 // the registers to keep the loads in flight; the other kinds keep theirs
 // (the full-block A^-1 tile runs 15% faster at three waves).
-constexpr int batch_max_waves(int p, int kind) { return p == P16 && kind == KA_CLOVER ? 2 : 8; }
+constexpr int batch_max_waves(int p, int kind) { return (p == P16 || p == PB16) && kind == KA_CLOVER ? 2 : 8; }
 
 template <int P, bool C12, int ISIGN, int KIND>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, batch_max_waves(P, KIND))))

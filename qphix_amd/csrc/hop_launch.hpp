@@ -33,7 +33,7 @@ template <int P>
 hipError_t launch_hop_prec(int kind, bool c12, int isign, const HopArgs &a, int block, hipStream_t st) {
   if (block <= 0 || block > 256 || (block & 63)) return hipErrorInvalidValue;
   if (c12) return isign > 0 ? launch_hop3<P, true, 1>(kind, a, block, st) : launch_hop3<P, true, -1>(kind, a, block, st);
-  if constexpr (P != P16) {
+  if constexpr (!PT<P>::scaled) {
     return isign > 0 ? launch_hop3<P, false, 1>(kind, a, block, st) : launch_hop3<P, false, -1>(kind, a, block, st);
   }
   return hipErrorInvalidValue;
@@ -67,7 +67,7 @@ template <int P>
 hipError_t launch_hop_tile_prec(int kind, bool c12, int isign, int rt, const HopArgs &a, int block, hipStream_t st) {
   if (block <= 0 || block > 256 || (block & 63) || a.nrhs % rt) return hipErrorInvalidValue;
   if (c12) return launch_tile2<P, true>(kind, isign, rt, a, block, st);
-  if constexpr (P != P16) return launch_tile2<P, false>(kind, isign, rt, a, block, st);
+  if constexpr (!PT<P>::scaled) return launch_tile2<P, false>(kind, isign, rt, a, block, st);
   return hipErrorInvalidValue;
 }
 
@@ -106,7 +106,7 @@ hipError_t launch_hop_batch_tile_prec(int kind, bool c12, int isign, const HopSe
   if (c12)
     return isign > 0 ? launch_batch_tile3<P, true, 1>(kind, segs, off, nseg, active, grid, st)
                      : launch_batch_tile3<P, true, -1>(kind, segs, off, nseg, active, grid, st);
-  if constexpr (P != P16) {
+  if constexpr (!PT<P>::scaled) {
     return isign > 0 ? launch_batch_tile3<P, false, 1>(kind, segs, off, nseg, active, grid, st)
                      : launch_batch_tile3<P, false, -1>(kind, segs, off, nseg, active, grid, st);
   }
@@ -120,7 +120,7 @@ hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *se
   if (c12)
     return isign > 0 ? launch_batch3<P, true, 1>(kind, segs, off, nseg, active, grid, st)
                      : launch_batch3<P, true, -1>(kind, segs, off, nseg, active, grid, st);
-  if constexpr (P != P16) {
+  if constexpr (!PT<P>::scaled) {
     return isign > 0 ? launch_batch3<P, false, 1>(kind, segs, off, nseg, active, grid, st)
                      : launch_batch3<P, false, -1>(kind, segs, off, nseg, active, grid, st);
   }

@@ -24,6 +24,7 @@ def test_geometry_validation_without_gpu():
     assert b"even" in lib.qphix_last_error()
     # half precision needs two-row compression
     assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), q.F16, 0, C.byref(h)) == q.EDOMAIN
+    assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), q.BF16, 0, C.byref(h)) == q.EDOMAIN
     assert lib.qphix_geom_create((C.c_int * 4)(8, 8, 8, 8), 7, 0, C.byref(h)) == q.EDOMAIN
     assert lib.qphix_geom_create(None, q.F64, 0, C.byref(h)) == q.EINVAL
     # parities beyond the kernels' 32-bit site index
@@ -36,6 +37,7 @@ def test_geometry_validation_without_gpu():
     (q.F32, 0, (96, 576, 288, 576)),
     (q.F32, 1, (96, 384, 288, 576)),
     (q.F16, 1, (52, 196, 148, 292)),
+    (q.BF16, 1, (52, 196, 148, 292)),
 ])
 def test_field_sizes(prec, c12, expect):
     g = q.Geometry([8, 8, 8, 8], prec, c12)

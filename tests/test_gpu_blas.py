@@ -18,11 +18,11 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
 
 
-@pytest.fixture(scope="module", params=[q.F64, q.F32, q.F16], ids=["f64", "f32", "f16"])
+@pytest.fixture(scope="module", params=[q.F64, q.F32, q.F16, q.BF16], ids=["f64", "f32", "f16", "bf16"])
 def env(request):
     prec = request.param
-    g = q.Geometry(L, prec, prec == q.F16)
-    tol = {q.F64: 1e-13, q.F32: 3e-7, q.F16: 2e-3}[prec]
+    g = q.Geometry(L, prec, prec in (q.F16, q.BF16))
+    tol = {q.F64: 1e-13, q.F32: 3e-7, q.F16: 2e-3, q.BF16: 1.6e-2}[prec]
     return g, tol
 
 
@@ -129,7 +129,7 @@ def test_blas_ops(env):
     assert rel(t.download(), ref) < tol * 4
     # copy / zero
     q.copy(g, out, x)
-    assert rel(out.download(), xh) < 1e-16 + tol * (g.prec == q.F16)
+    assert rel(out.download(), xh) < 1e-16 + tol * (g.prec in (q.F16, q.BF16))
     q.zero(g, out)
     assert not out.download().any()
 

@@ -15,7 +15,7 @@ import qphix_amd as q
 
 pytestmark = pytest.mark.gpu
 
-TOL = {q.F64: 1e-13, q.F32: 2e-6, q.F16: 5e-3}
+TOL = {q.F64: 1e-13, q.F32: 2e-6, q.F16: 5e-3, q.BF16: 4e-2}
 L8 = [8, 8, 8, 8]
 
 
@@ -45,8 +45,8 @@ class Setup:
 
 
 @pytest.fixture(scope="module", params=[(q.F64, False), (q.F64, True), (q.F32, False),
-                                        (q.F32, True), (q.F16, True)],
-                ids=["f64c18", "f64c12", "f32c18", "f32c12", "f16c12"])
+                                        (q.F32, True), (q.F16, True), (q.BF16, True)],
+                ids=["f64c18", "f64c12", "f32c18", "f32c12", "f16c12", "bf16c12"])
 def s8(request):
     prec, c12 = request.param
     return Setup(L8, prec, c12)
@@ -98,7 +98,7 @@ def test_clover_dslash_and_achimbdpsi(s8, isign):
     rho = 0.05
     D = q.Dslash(s8.g, t_boundary=-1.0, prec_mass_rho=rho)
     hp = oracle.hop_params(-1.0, 1.0, 1.0)
-    tol = TOL[s8.prec] * (4 if s8.prec == q.F16 else 1)
+    tol = TOL[s8.prec] * (4 if s8.prec in (q.F16, q.BF16) else 1)
     for cb in (0, 1):
         psi, psih = s8.spinor(40, 1 - cb)
         chi, chih = s8.spinor(41, cb)
@@ -152,7 +152,7 @@ def test_tmclov_dslash_full_blocks(s8):
     mu = 0.06
     D = q.Dslash(s8.g, t_boundary=-1.0, prec_mass_rho=mu)
     hp = oracle.hop_params(-1.0, 1.0, 1.0)
-    tol = TOL[s8.prec] * (4 if s8.prec == q.F16 else 1)
+    tol = TOL[s8.prec] * (4 if s8.prec in (q.F16, q.BF16) else 1)
     for cb in (0, 1):
         fp = oracle.full_clover_tm_inverse(s8.clov[cb], mu)
         fm = oracle.full_clover_tm_inverse(s8.clov[cb], -mu)
@@ -170,7 +170,7 @@ def test_tmclov_dslash_full_blocks(s8):
 def test_two_flav_inverse_clover_term(s8):
     mu, eps = 0.05, 0.02
     D = q.Dslash(s8.g)
-    tol = TOL[s8.prec] * (4 if s8.prec == q.F16 else 1)
+    tol = TOL[s8.prec] * (4 if s8.prec in (q.F16, q.BF16) else 1)
     fu, fd, co = oracle.ndclover_inverse(s8.clov[0], mu, eps)
     fdev = [s8.g.full_clover().upload(fu), s8.g.full_clover().upload(fd)]
     cdev = s8.g.clover().upload(co)
@@ -307,7 +307,8 @@ def test_gpu_clover_build_closed_form(L, fluxes, rot, c12):
             assert np.abs(got - want).max() < 1e-13, what
 
 
-@pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True)], ids=["f64", "f32", "f16"])
+@pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True), (q.BF16, True)],
+                         ids=["f64", "f32", "f16", "bf16"])
 def test_clover_apply_entry_point(prec, c12):
     """qphix_clover_apply: A x with packed Hermitian blocks and F x with full
     complex blocks (the site product of clover_product.h)."""
@@ -322,7 +323,7 @@ def test_clover_apply_entry_point(prec, c12):
     cd = g.clover().upload(clov)
     fd = g.full_clover().upload(fcl)
     out = g.spinor()
-    tol = TOL[prec] * (4 if prec == q.F16 else 1)
+    tol = TOL[prec] * (4 if prec in (q.F16, q.BF16) else 1)
     q.call("qphix_clover_apply", g.h, out.p, x.p, cd.p, 0)
     assert rel(out.download(0), oracle.clover_apply(clov, xh)) < tol
     q.call("qphix_clover_apply", g.h, out.p, x.p, fd.p, 1)
@@ -336,9 +337,10 @@ def test_precision_conversion():
     g64 = q.Geometry(L, q.F64, False)
     g32 = q.Geometry(L, q.F32, True)
     g16 = q.Geometry(L, q.F16, True)
+    gb16 = q.Geometry(L, q.BF16, True)
     xh = oracle.gen_spinor(L, 4, 0)
     x = g64.spinor().upload(xh)
-    for gl, tol in ((g32, 1e-7), (g16, 2e-3)):
+    for gl, tol in ((g32, 1e-7), (g16, 2e-3), (gb16, 1.6e-2)):
         y = gl.spinor()
         q.convert_spinor(gl, y, 0.5, g64, x)
         z = g64.spinor()
@@ -347,7 +349,7 @@ def test_precision_conversion():
     # gauge conversion F64 c18 -> F32/F16 c12 preserves the hop to the storage precision
     u = oracle.gen_gauge(L, 2, 0.4)
     ud = q.pack_gauge(g64, u)
-    for gl, tol in ((g32, 2e-6), (g16, 5e-3)):
+    for gl, tol in ((g32, 2e-6), (g16, 5e-3), (gb16, 4e-2)):
         ul = q.convert_gauge(gl, g64, ud)
         psi = gl.spinor()
         q.convert_spinor(gl, psi, 1.0, g64, g64.spinor().upload(oracle.gen_spinor(L, 6, 0)))

@@ -13,9 +13,9 @@ import qphix_amd as q
 
 pytestmark = pytest.mark.gpu
 
-TOL = {q.F64: 1e-13, q.F32: 2e-6, q.F16: 5e-3}
+TOL = {q.F64: 1e-13, q.F32: 2e-6, q.F16: 5e-3, q.BF16: 4e-2}
 SHAPES = [[2, 2, 2, 2], [6, 2, 2, 4], [2, 4, 2, 6], [10, 6, 2, 2], [14, 6, 4, 2], [6, 10, 6, 4]]
-PRECS = [(q.F64, False), (q.F32, True), (q.F16, True)]
+PRECS = [(q.F64, False), (q.F32, True), (q.F16, True), (q.BF16, True)]
 
 
 def rel(a, b):
@@ -32,7 +32,7 @@ def lat(request):
     return L, vh, u, cl, [oracle.clover_invert(c) for c in cl]
 
 
-@pytest.mark.parametrize("prec,c12", PRECS, ids=["f64", "f32c12", "f16c12"])
+@pytest.mark.parametrize("prec,c12", PRECS, ids=["f64", "f32c12", "f16c12", "bf16c12"])
 def test_kernels(lat, prec, c12):
     L, vh, u, cl, inv = lat
     g = q.Geometry(L, prec, c12)

@@ -104,7 +104,8 @@ def forced_rhs_tiling():
 
 
 @pytest.mark.parametrize("reg,rt", [(1, 4), (2, 2), (2, 6), (4, 4), (4, 8), (8, 8), (8, 16)])
-@pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True)], ids=["f64", "f32c12", "f16c12"])
+@pytest.mark.parametrize("prec,c12", [(q.F64, False), (q.F32, True), (q.F16, True), (q.BF16, True)],
+                         ids=["f64", "f32c12", "f16c12", "bf16c12"])
 def test_register_tiled_multi_rhs(forced_rhs_tiling, reg, rt, prec, c12):
     """Each forced tiling (register tile reg, rt sources per launch; n = 5 or,
     for the matrix-core kernel, 11, so that a remainder goes through the

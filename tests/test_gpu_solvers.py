@@ -181,13 +181,13 @@ def test_minvcg_too_many_shifts(world):
     assert e.value.code == q.EDOMAIN
 
 
-@pytest.mark.parametrize("inner_prec", [q.F32, q.F16], ids=["f32", "f16"])
+@pytest.mark.parametrize("inner_prec", [q.F32, q.F16, q.BF16], ids=["f32", "f16", "bf16"])
 def test_richardson_mixed_clover(world, inner_prec):
     """FP64 defect correction around a low-precision BiCGStab (config 3 at 8^4)."""
     g, ops, _ = world.device(q.F64, False)
     gi, opsi, _ = world.device(inner_prec, True)
     inner = q.InvBiCGStab(opsi["clover"], 500)
-    solver = q.InvRichardsonMultiPrec(ops["clover"], inner, 1e-2 if inner_prec == q.F16 else 1e-4,
+    solver = q.InvRichardsonMultiPrec(ops["clover"], inner, 1e-2 if inner_prec in (q.F16, q.BF16) else 1e-4,
                                       200)
     b, bh = flavours(g, 1, 150, 1)
     x = g.spinor()
