@@ -424,6 +424,12 @@ class Solver:
         except Exception:
             pass
 
+    def set_mode(self, mode):
+        """0 host scalars, 1 device scalars, 2 device scalars with the
+        iteration replayed from a HIP graph (BiCGStab and multishift CG)."""
+        call("qphix_solver_set_mode", self.h, mode)
+        return self
+
     def __call__(self, x, rhs, rsd_target, isign=1, target_cb=1, resid=RESID_RELATIVE,
                  verbose=False):
         x = x if isinstance(x, (list, tuple)) else [x]

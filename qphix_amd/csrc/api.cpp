@@ -87,6 +87,9 @@ struct qphix_solver {
   // FP64 scalars in device memory with eager launches, 2 the same with each
   // iteration replayed from a HIP graph
   int ms_mode = 2;
+  // BiCGStab control: 0 host scalars, 1 device scalars (eager launches), 2
+  // device scalars with the iteration replayed from a HIP graph
+  int dev_mode = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -936,6 +939,11 @@ static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs 
   }
 }
 
+// set by the device-resident solver loops while they enqueue iterations: the
+// hop launches of their operator applications skip themselves once the
+// device state says the solve is done
+static const int *g_hop_skip = nullptr;
+
 static HopArgs make_hop_args(const qphix_dslash *d, int cb, const void *u, const void *clov, double coef,
                              double alpha, double mu, double muinv, double rho) {
   const GeomDesc &g = d->g->d;
@@ -965,6 +973,7 @@ static HopArgs make_hop_args(const qphix_dslash *d, int cb, const void *u, const
   a.rho = rho;
   a.slabs = 1;
   a.nrhs = 1;
+  a.skip = g_hop_skip;
   return a;
 }
 
@@ -1662,6 +1671,133 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
 // invbicgstab.h:86-262.  Counters as invbicgstab.h:158-240 counts them,
 // including the initial M x (skipped, not its count, when x is exactly zero);
 // the |r0|^2 of the iteration-0 test is not counted there either.
+// a BLAS step whose scalars come from device memory (dsc; BlasArgs::dsc), its
+// reduction (if any) finalized into device memory: kernels only
+static int run_blas_q(const qphix_geom *g, int op, BlasArgs a, const double *dsc, double *d_out) {
+  a.dsc = dsc;
+  int nb = 0;
+  HIPCHK(blas_launch(g->d, op, a, g_partials, &nb, g_stream));
+  if (blas_nred(op) > 0) HIPCHK(blas_finalize(g_partials, nb, blas_nred(op), d_out, g_stream));
+  return QPHIX_OK;
+}
+
+struct GraphRun {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~GraphRun() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+static MsState::HostView *g_bi_view = nullptr;  // pinned host, 2 entries
+
+// The BiCGStab iteration (single flavour) with its scalars in device memory
+// (BiState): p-update, v = M p, <r0, v>, s-update, t = M r, <t, r> and |t|^2,
+// the x/r update with |r|^2 and <r0, r>, between them four single-thread
+// kernels with the host loop's scalar arithmetic; no host round trip. Mode 2
+// captures one iteration in a HIP graph and replays it. The host enqueues
+// iteration k + 1 before it reads iteration k's completion flag, except when
+// the residual trend says k may be the last; iterations after the end only
+// read (the vector updates skip themselves).
+static int bicgstab_device_loop(qphix_solver *s, void *x, const SV &r0, const SV &r, const SV &p, SV &v, SV &t,
+                                int isign, int cb, double r_norm, double rsd_sq, const double r0r[2],
+                                unsigned long sf, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  if (!g_bi_view) HIPCHK(hipHostMalloc(&g_bi_view, 2 * sizeof(MsState::HostView), hipHostMallocCoherent));
+  BiState h;
+  memset(&h, 0, sizeof(h));
+  h.rho_prev[0] = h.alpha[0] = h.omega[0] = 1.0;
+  h.ru[1] = r0r[0];
+  h.ru[2] = r0r[1];
+  h.r_norm = r_norm;
+  h.rsd_sq = rsd_sq;
+  h.site_flops = st->site_flops;
+  h.mv_apps = st->mv_apps;
+  h.sf = sf;
+  h.max_iters = s->max_iters;
+  h.progress = r_norm / rsd_sq;
+  h.hview = g_bi_view;
+  DevBuf dbs;
+  if (hipMalloc(&dbs.p, sizeof(BiState)) != hipSuccess) return fail(QPHIX_ENOMEM, "BiCGStab state allocation");
+  BiState *bs = (BiState *)dbs.p;
+  HIPCHK(hipMemcpyAsync(bs, &h, sizeof(h), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  struct SkipScope {  // hop launches of this loop skip once the solve is done
+    explicit SkipScope(const int *p) { g_hop_skip = p; }
+    ~SkipScope() { g_hop_skip = nullptr; }
+  } skip_scope(&bs->done);
+  auto iteration = [&]() -> int {
+    HIPCHK(bi_pre_launch(bs, g_stream));
+    BlasArgs a = bargs();
+    a.x = r[0]; a.y = v[0]; a.o0 = p[0];
+    TRY(run_blas_q(g, B_PUPD, a, bs->pupd, nullptr));
+    TRY(op_apply_v(M, v, cv(p), isign, cb));
+    a = bargs();
+    a.x = r0[0]; a.y = v[0];
+    TRY(run_blas_q(g, B_CDOT, a, nullptr, bs->r0v));
+    HIPCHK(bi_mid_launch(bs, g_stream));
+    a = bargs();
+    a.x = v[0]; a.o0 = r[0];
+    TRY(run_blas_q(g, B_SUPD, a, bs->supd, nullptr));
+    TRY(op_apply_v(M, t, cv(r), isign, cb));
+    a = bargs();
+    a.x = t[0]; a.y = r[0];
+    TRY(run_blas_q(g, B_CDOTNORM, a, nullptr, bs->tr));
+    HIPCHK(bi_mid2_launch(bs, g_stream));
+    a = bargs();
+    a.x = t[0]; a.y = p[0]; a.z = r0[0]; a.o0 = x; a.o1 = r[0];
+    TRY(run_blas_q(g, B_RXUPD_CDOT, a, bs->rxupd, bs->ru));
+    HIPCHK(bi_post_launch(bs, g_stream));
+    return QPHIX_OK;
+  };
+  GraphRun gr;
+  HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
+  double prev = -1, prev2 = -1;  // residual ratio of the last two iterations read
+  for (int k = 0;; k++) {
+    // the first iteration runs eagerly (it tunes any kernel not tuned yet),
+    // mode 2 captures the second one and replays it from then on
+    if (s->dev_mode == 2 && k == 1) {
+      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      const int rc = iteration();
+      const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
+      if (rc) return rc;
+      HIPCHK(e);
+      HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+    }
+    if (s->dev_mode == 2 && k >= 1) {
+      HIPCHK(hipGraphLaunch(gr.exec, g_stream));
+    } else {
+      TRY(iteration());
+    }
+    HIPCHK(hipEventRecord(gr.ev[k & 1], g_stream));
+    if (k > 0) {
+      HIPCHK(hipEventSynchronize(gr.ev[(k - 1) & 1]));
+      if (g_bi_view[(k - 1) & 1].done) break;
+      prev2 = prev;
+      prev = g_bi_view[(k - 1) & 1].progress;
+    }
+    const bool last_near = (prev > 0 && prev2 > 0 && prev * (prev / prev2) < 1.5) || k + 1 >= s->max_iters;
+    if (last_near) {
+      HIPCHK(hipEventSynchronize(gr.ev[k & 1]));
+      if (g_bi_view[k & 1].done) break;
+    }
+    if (k > s->max_iters + 1) break;  // the state is done after max_iters iterations
+  }
+  HIPCHK(hipMemcpyAsync(&h, bs, sizeof(h), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  st->site_flops = h.site_flops;
+  st->mv_apps = h.mv_apps;
+  st->rsd_sq_final = h.r_norm;
+  st->n_iters = h.n_iters;
+  st->status = h.status;
+  return QPHIX_OK;
+}
+
 static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                           int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -1717,6 +1853,8 @@ static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rh
   // of the previous iteration's x/r update
   double r0r[2];
   TRY(mf_cdot(g, nf, cv(r0), cv(r), r0r));
+  if (s->dev_mode > 0 && nf == 1 && s->max_iters >= 1 && !verbose)
+    return bicgstab_device_loop(s, x[0], r0, r, p, v, t, isign, cb, r_norm, rsd_sq, r0r, sf, st);
   for (k = 1; k <= s->max_iters && notconv; k++) {
     rho[0] = r0r[0];
     rho[1] = r0r[1];
@@ -2579,6 +2717,10 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
     }
     return QPHIX_OK;
   };
+  struct SkipScope {  // hop launches of this loop skip once the solve is done
+    explicit SkipScope(const int *p) { g_hop_skip = p; }
+    ~SkipScope() { g_hop_skip = nullptr; }
+  } skip_scope(&ms->done);
   auto iteration = [&]() -> int {
     TRY(sweep());
     TRY(op_apply_v(M, mp, (const void *const *)p0.data(), isign, cb));
@@ -2598,17 +2740,19 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
   MsGraph gr;
   HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
-  if (s->ms_mode == 2) {
-    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-    const int rc = iteration();
-    const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
-    if (rc) return rc;
-    HIPCHK(e);
-    HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
-  }
   double prev = -1, prev2 = -1;  // progress of the last two iterations read
   for (int k = 0;; k++) {
-    if (s->ms_mode == 2) {
+    // the first iteration runs eagerly (it tunes any kernel not tuned yet),
+    // mode 2 captures the second one and replays it from then on
+    if (s->ms_mode == 2 && k == 1) {
+      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      const int rc = iteration();
+      const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
+      if (rc) return rc;
+      HIPCHK(e);
+      HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+    }
+    if (s->ms_mode == 2 && k >= 1) {
       HIPCHK(hipGraphLaunch(gr.exec, g_stream));
     } else {
       TRY(iteration());
@@ -2640,6 +2784,19 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
   if (mv_apps) *mv_apps = 2ul + 2ul * (unsigned long)h.iters;
   *k_out = h.iters + 1;
   return h.converged ? QPHIX_OK : -1;
+}
+
+extern "C" int qphix_solver_set_mode(qphix_solver *s, int mode) {
+  CHKPTR(s);
+  if (mode < 0 || mode > 2) return fail(QPHIX_EDOMAIN, "solver mode %d (0, 1 or 2)", mode);
+  if (s->kind == S_MINVCG) {
+    s->ms_mode = mode;
+  } else if (s->kind == S_BICGSTAB) {
+    s->dev_mode = mode;
+  } else {
+    return fail(QPHIX_EINVAL, "device-resident control exists for BiCGStab and multishift CG solvers");
+  }
+  return QPHIX_OK;
 }
 
 extern "C" int qphix_minvcg_set_mode(qphix_solver *s, int mode) {

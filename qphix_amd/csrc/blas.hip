@@ -274,8 +274,14 @@ template <int NR> __device__ inline void block_reduce_store(double *red, double 
 }
 
 template <int P, int OP>
-__global__ void __launch_bounds__(kBlasThreads) k_blas(const BlasArgs a, size_t vs, long vh, double *partials) {
+__global__ void __launch_bounds__(kBlasThreads) k_blas(const BlasArgs a0, size_t vs, long vh, double *partials) {
   constexpr int NR = nred_of(OP);
+  BlasArgs a = a0;
+  if (a0.dsc) {
+    if (a0.dsc[7] != 0.0) return;
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.a[k] = a0.dsc[k];
+  }
   double red[NR > 0 ? NR : 1] = {0};
   const long stride = (long)gridDim.x * blockDim.x;
   for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < vh; s += stride) blas_site<P, OP>(a, vs, s, red);
@@ -560,6 +566,147 @@ hipError_t ms_mid_launch(MsState *st_dev, hipStream_t st) {
 }
 hipError_t ms_post_launch(MsState *st_dev, hipStream_t st) {
   k_ms_post<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab with its scalars in device memory (invbicgstab.h:150-256)
+//
+// Each kernel evaluates the host loop's expressions in the same order with
+// contraction off (the host's flavour sums start from 0, so do these), so
+// the solve is bit-identical to the one with host scalars. A breakdown or the
+// end of the solve freezes the state and sets the skip flag of the vector
+// updates.
+// ---------------------------------------------------------------------------
+__device__ inline void bi_cdiv(double *o, const double *a, const double *b) {
+#pragma clang fp contract(off)
+  const double den = b[0] * b[0] + b[1] * b[1];
+  const double re = (a[0] * b[0] + a[1] * b[1]) / den, im = (a[1] * b[0] - a[0] * b[1]) / den;
+  o[0] = re;
+  o[1] = im;
+}
+__device__ inline void bi_cmul(double *o, const double *a, const double *b) {
+#pragma clang fp contract(off)
+  const double re = a[0] * b[0] - a[1] * b[1], im = a[0] * b[1] + a[1] * b[0];
+  o[0] = re;
+  o[1] = im;
+}
+__device__ inline void bi_freeze(BiState *b, int status, int n_iters) {
+  b->status = status;
+  b->n_iters = n_iters;
+  b->done = 1;
+  b->pupd[7] = b->supd[7] = b->rxupd[7] = 1.0;
+}
+__device__ inline void bi_publish(BiState *b) {
+  MsState::HostView *v = b->hview + (b->posts & 1);
+  v->done = b->done;
+  v->progress = b->progress;
+  __threadfence_system();
+  b->posts++;
+}
+
+__global__ void k_bi_pre(BiState *b) {
+#pragma clang fp contract(off)
+  if (b->done) return;
+  const int k = b->iters + 1;
+  double rho[2] = {0, 0};
+  rho[0] += b->ru[1];
+  rho[1] += b->ru[2];
+  b->rho[0] = rho[0];
+  b->rho[1] = rho[1];
+  b->site_flops += 8 * b->sf;
+  if (rho[0] == 0 && rho[1] == 0) {
+    bi_freeze(b, 4 /* QPHIX_EBREAKDOWN */, k);
+    return;
+  }
+  double t1[2], t2[2], beta[2];
+  bi_cdiv(t1, rho, b->rho_prev);
+  bi_cdiv(t2, b->alpha, b->omega);
+  bi_cmul(beta, t1, t2);
+  b->pupd[0] = beta[0];
+  b->pupd[1] = beta[1];
+  b->pupd[2] = b->omega[0];
+  b->pupd[3] = b->omega[1];
+  b->site_flops += 16 * b->sf;
+}
+
+__global__ void k_bi_mid(BiState *b) {
+#pragma clang fp contract(off)
+  if (b->done) return;
+  const int k = b->iters + 1;
+  b->mv_apps++;
+  double tmp[2] = {0, 0};
+  tmp[0] += b->r0v[0];
+  tmp[1] += b->r0v[1];
+  b->site_flops += 8 * b->sf;
+  if (tmp[0] == 0 && tmp[1] == 0) {
+    bi_freeze(b, 4, k);
+    return;
+  }
+  bi_cdiv(b->alpha, b->rho, tmp);
+  b->rho_prev[0] = b->rho[0];
+  b->rho_prev[1] = b->rho[1];
+  b->supd[0] = b->alpha[0];
+  b->supd[1] = b->alpha[1];
+  b->site_flops += 8 * b->sf;
+}
+
+__global__ void k_bi_mid2(BiState *b) {
+#pragma clang fp contract(off)
+  if (b->done) return;
+  const int k = b->iters + 1;
+  b->mv_apps++;
+  double tr[3] = {0, 0, 0};
+  for (int q = 0; q < 3; q++) tr[q] += b->tr[q];
+  b->site_flops += 4 * b->sf;
+  const double t_norm = tr[2];
+  if (t_norm == 0) {
+    bi_freeze(b, 4, k);
+    return;
+  }
+  b->site_flops += 8 * b->sf;
+  b->omega[0] = tr[0] / t_norm;
+  b->omega[1] = tr[1] / t_norm;
+  b->rxupd[0] = b->omega[0];
+  b->rxupd[1] = b->omega[1];
+  b->rxupd[2] = b->alpha[0];
+  b->rxupd[3] = b->alpha[1];
+}
+
+__global__ void k_bi_post(BiState *b) {
+#pragma clang fp contract(off)
+  if (b->done) {
+    bi_publish(b);
+    return;
+  }
+  const int k = ++b->iters;
+  b->site_flops += 28 * b->sf;
+  double r_norm = 0;
+  r_norm += b->ru[0];
+  b->r_norm = r_norm;
+  b->progress = r_norm / b->rsd_sq;
+  if (r_norm < b->rsd_sq) {
+    bi_freeze(b, 0, k + 1);
+  } else if (k >= b->max_iters) {
+    bi_freeze(b, 5 /* QPHIX_ENOCONV */, k + 1);
+  }
+  bi_publish(b);
+}
+
+hipError_t bi_pre_launch(BiState *st_dev, hipStream_t st) {
+  k_bi_pre<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+hipError_t bi_mid_launch(BiState *st_dev, hipStream_t st) {
+  k_bi_mid<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+hipError_t bi_mid2_launch(BiState *st_dev, hipStream_t st) {
+  k_bi_mid2<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+hipError_t bi_post_launch(BiState *st_dev, hipStream_t st) {
+  k_bi_post<<<1, 1, 0, st>>>(st_dev);
   return hipGetLastError();
 }
 

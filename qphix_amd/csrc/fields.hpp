@@ -54,6 +54,10 @@ struct BlasArgs {
   const void *x, *y, *z, *w;  // read-only operands
   void *o0, *o1, *o2, *o3;    // written (and possibly read) operands
   double a[6];                // scalars
+  // non-null: the scalars are read from device memory, dsc[0..5], when the
+  // kernel starts, and the launch does nothing if dsc[7] != 0 (a solver
+  // whose device state says it is done)
+  const double *dsc;
 };
 int blas_nred(int op);
 // launches op over the geometry; writes nred partial sums per block to
@@ -115,6 +119,30 @@ hipError_t ms_mid_launch(MsState *st_dev, hipStream_t st);
 // the start of the next iteration as ms_pre_launch (or, once the solve is
 // done, only the pending x_s updates for the next sweep)
 hipError_t ms_post_launch(MsState *st_dev, hipStream_t st);
+// BiCGStab state in device memory (invbicgstab.h:150-256): the complex
+// scalars, the reductions the device loop writes (<r0, v>, <t, r> with |t|^2,
+// |r|^2 with <r0, r>), and the scalar blocks of the three vector updates
+// (BlasArgs::dsc; [7] = skip once done). Updated by single-thread kernels,
+// so one iteration needs no host round trip.
+struct BiState {
+  double rho[2], rho_prev[2], alpha[2], omega[2];
+  double r0v[2], tr[3], ru[3];
+  double r_norm, rsd_sq;
+  double pupd[8], supd[8], rxupd[8];
+  unsigned long site_flops, mv_apps, sf;  // sf: the reference's unit, 12 x flavours
+  int iters, max_iters, done, status, n_iters, posts;
+  double progress;
+  MsState::HostView *hview;
+};
+// start of iteration k: rho = <r0, r>, the breakdown test, beta, p-update scalars
+hipError_t bi_pre_launch(BiState *st_dev, hipStream_t st);
+// after v = M p and <r0, v>: the breakdown test, alpha, s-update scalars
+hipError_t bi_mid_launch(BiState *st_dev, hipStream_t st);
+// after t = M r and <t, r>, |t|^2: the breakdown test, omega, x/r-update scalars
+hipError_t bi_mid2_launch(BiState *st_dev, hipStream_t st);
+// after the x/r update: |r|^2, convergence, the iteration count
+hipError_t bi_post_launch(BiState *st_dev, hipStream_t st);
+
 // y += (*b) x and |y|^2, the scalar read from device memory; partials as
 // blas_launch, the reduction finalized into *out
 hipError_t axpy_norm2_dev_launch(const GeomDesc &g, const double *b, const void *x, void *y, double *partials,

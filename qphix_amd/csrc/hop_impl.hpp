@@ -53,6 +53,9 @@ struct HopArgs {
   // the work-groups resident per CU, so the sites in flight on one XCD, whose
   // neighbour spinors must stay in its L2, fit it (0: no cap)
   int lds_cap;
+  // non-null: the launch does nothing when *skip != 0 (a solver iteration
+  // enqueued after its device state says the solve is done)
+  const int *skip;
   const void *psis[kMaxRhs];
   void *outs[kMaxRhs];
   const void *chis[kMaxRhs];
@@ -759,6 +762,7 @@ constexpr int hop_min_waves(int p) { return (p == P16 || p == PB16) ? QPHIX_HOP_
 
 template <int P, bool C12, int ISIGN, int KIND>
 __global__ void __launch_bounds__(256, hop_min_waves(P)) hop_kernel(const HopArgs a) {
+  if (a.skip && *a.skip) return;
   int lb = xcd_block();
   const void *psi_p = a.psi, *chi_p = a.chi;
   void *out_p = a.out;
@@ -788,6 +792,7 @@ __global__ void __launch_bounds__(256, hop_min_waves(P)) hop_kernel(const HopArg
 // together and also share links and clover through the L2.
 template <int P, bool C12, int ISIGN, int KIND, int R>
 __global__ void __launch_bounds__(256) hop_tile_kernel(const HopArgs a) {
+  if (a.skip && *a.skip) return;
   int lb = xcd_block();
   const int ng = a.nrhs / R;
   const int g = lb % ng;

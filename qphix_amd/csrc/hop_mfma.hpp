@@ -159,6 +159,7 @@ __global__ void __launch_bounds__(256) hop_mfma_kernel(const HopArgs a) {
   __shared__ float lh[kMfmaSites][kMfmaRhs][12];
   __shared__ float lu[kMfmaSites][18];
   __shared__ float lw[kMfmaSites][kMfmaRhs][12];
+  if (a.skip && *a.skip) return;
   int lb = xcd_block();
   const int ng = a.nrhs / kMfmaRhs;
   const int g = lb % ng;
