@@ -356,11 +356,12 @@ int qphix_solve_multi(qphix_solver *s, void *const x[], const void *const rhs[],
  * one iteration captured in a HIP graph and replayed. All three give
  * bit-identical solutions and counters. */
 int qphix_minvcg_set_mode(qphix_solver *s, int mode);
-/* The same control for BiCGStab solvers (no reference counterpart; the
- * reference keeps its scalars on the host, invbicgstab.h:150-256) and for
- * multishift CG (as qphix_minvcg_set_mode). BiCGStab default 0; modes 1 and 2
- * apply to single-flavour operators and silent solves (the others run mode 0)
- * and give bit-identical solutions, counters and statuses. */
+/* The same control for CG and BiCGStab solvers (no reference counterpart;
+ * the reference keeps their scalars on the host, invcg.h:293-343,
+ * invbicgstab.h:150-256) and for multishift CG (as qphix_minvcg_set_mode). CG
+ * and BiCGStab default to 0; modes 1 and 2 apply to single-flavour operators
+ * and silent solves (the others run mode 0) and give bit-identical solutions,
+ * counters and statuses. */
 int qphix_solver_set_mode(qphix_solver *s, int mode);
 /* x[shift * nflav + f]; rsd_sq_final[n_shift] */
 int qphix_minvcg_solve(qphix_solver *s, void *const x[], const void *const rhs[],

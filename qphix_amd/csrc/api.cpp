@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -1586,6 +1587,10 @@ extern "C" int qphix_solver_destroy(qphix_solver *s) {
 // when x is exactly zero.  On convergence the reference leaves rsd_sq_final
 // unset; here it is the recursively updated |r|^2 (absolute), the quantity the
 // convergence test used.
+static int run_blas_q(const qphix_geom *g, int op, BlasArgs a, const double *dsc, double *d_out);
+static int cg_device_loop(qphix_solver *s, void *x, const SV &r, const SV &p, SV &mp, SV &mmp, int isign, int cb,
+                          double cp, double rsd_sq, unsigned long sf, qphix_solve_stats *st);
+
 static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                     int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -1636,6 +1641,7 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
     return QPHIX_OK;
   }
   for (int f = 0; f < nf; f++) TRY(qphix_copy_spinor(g, p[f], r[f]));
+  if (s->dev_mode > 0 && nf == 1 && !verbose) return cg_device_loop(s, x[0], r, p, mp, mmp, isign, cb, cp, rsd_sq, sf, st);
   for (int k = 0; k <= s->max_iters; ++k) {
     const double c = cp;
     TRY(op_apply_v(M, mp, cv(p), isign, cb));
@@ -1668,9 +1674,6 @@ static int solve_cg(qphix_solver *s, void *const *x, const void *const *rhs, dou
   return QPHIX_OK;
 }
 
-// invbicgstab.h:86-262.  Counters as invbicgstab.h:158-240 counts them,
-// including the initial M x (skipped, not its count, when x is exactly zero);
-// the |r0|^2 of the iteration-0 test is not counted there either.
 // a BLAS step whose scalars come from device memory (dsc; BlasArgs::dsc), its
 // reduction (if any) finalized into device memory: kernels only
 static int run_blas_q(const qphix_geom *g, int op, BlasArgs a, const double *dsc, double *d_out) {
@@ -1694,14 +1697,56 @@ struct GraphRun {
 };
 static MsState::HostView *g_bi_view = nullptr;  // pinned host, 2 entries
 
+// Runs `iteration` (kernels only, one solver iteration on g_stream) until the
+// device state it publishes to view[posts & 1] says done. The host enqueues
+// iteration k + 1 before it reads iteration k's view, so the GPU does not wait
+// for the host, except when the residual ratio's trend says k may be the last
+// (or the step limit is near): then it waits for k first. The first iteration
+// runs eagerly (it tunes any kernel not tuned yet); mode 2 captures the second
+// one in a HIP graph and replays it from then on.
+static int device_iterations(int mode, int max_steps, const std::function<int()> &iteration,
+                             MsState::HostView *view) {
+  GraphRun gr;
+  HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
+  double prev = -1, prev2 = -1;  // residual ratio of the last two iterations read
+  for (int k = 0;; k++) {
+    if (mode == 2 && k == 1) {
+      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      const int rc = iteration();
+      const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
+      if (rc) return rc;
+      HIPCHK(e);
+      HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+    }
+    if (mode == 2 && k >= 1) {
+      HIPCHK(hipGraphLaunch(gr.exec, g_stream));
+    } else {
+      TRY(iteration());
+    }
+    HIPCHK(hipEventRecord(gr.ev[k & 1], g_stream));
+    if (k > 0) {
+      HIPCHK(hipEventSynchronize(gr.ev[(k - 1) & 1]));
+      if (view[(k - 1) & 1].done) break;
+      prev2 = prev;
+      prev = view[(k - 1) & 1].progress;
+    }
+    const bool last_near = (prev > 0 && prev2 > 0 && prev * (prev / prev2) < 1.5) || k + 2 >= max_steps;
+    if (last_near) {
+      HIPCHK(hipEventSynchronize(gr.ev[k & 1]));
+      if (view[k & 1].done) break;
+    }
+    if (k > max_steps + 1) break;  // the device state is done by then
+  }
+  return QPHIX_OK;
+}
+
 // The BiCGStab iteration (single flavour) with its scalars in device memory
 // (BiState): p-update, v = M p, <r0, v>, s-update, t = M r, <t, r> and |t|^2,
 // the x/r update with |r|^2 and <r0, r>, between them four single-thread
-// kernels with the host loop's scalar arithmetic; no host round trip. Mode 2
-// captures one iteration in a HIP graph and replays it. The host enqueues
-// iteration k + 1 before it reads iteration k's completion flag, except when
-// the residual trend says k may be the last; iterations after the end only
-// read (the vector updates skip themselves).
+// kernels with the host loop's scalar arithmetic; no host round trip
+// (device_iterations runs it). Iterations after the end only read: the vector
+// updates and the hops skip themselves.
 static int bicgstab_device_loop(qphix_solver *s, void *x, const SV &r0, const SV &r, const SV &p, SV &v, SV &t,
                                 int isign, int cb, double r_norm, double rsd_sq, const double r0r[2],
                                 unsigned long sf, qphix_solve_stats *st) {
@@ -1754,40 +1799,7 @@ static int bicgstab_device_loop(qphix_solver *s, void *x, const SV &r0, const SV
     HIPCHK(bi_post_launch(bs, g_stream));
     return QPHIX_OK;
   };
-  GraphRun gr;
-  HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
-  double prev = -1, prev2 = -1;  // residual ratio of the last two iterations read
-  for (int k = 0;; k++) {
-    // the first iteration runs eagerly (it tunes any kernel not tuned yet),
-    // mode 2 captures the second one and replays it from then on
-    if (s->dev_mode == 2 && k == 1) {
-      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-      const int rc = iteration();
-      const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
-      if (rc) return rc;
-      HIPCHK(e);
-      HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
-    }
-    if (s->dev_mode == 2 && k >= 1) {
-      HIPCHK(hipGraphLaunch(gr.exec, g_stream));
-    } else {
-      TRY(iteration());
-    }
-    HIPCHK(hipEventRecord(gr.ev[k & 1], g_stream));
-    if (k > 0) {
-      HIPCHK(hipEventSynchronize(gr.ev[(k - 1) & 1]));
-      if (g_bi_view[(k - 1) & 1].done) break;
-      prev2 = prev;
-      prev = g_bi_view[(k - 1) & 1].progress;
-    }
-    const bool last_near = (prev > 0 && prev2 > 0 && prev * (prev / prev2) < 1.5) || k + 1 >= s->max_iters;
-    if (last_near) {
-      HIPCHK(hipEventSynchronize(gr.ev[k & 1]));
-      if (g_bi_view[k & 1].done) break;
-    }
-    if (k > s->max_iters + 1) break;  // the state is done after max_iters iterations
-  }
+  TRY(device_iterations(s->dev_mode, s->max_iters + 1, iteration, g_bi_view));
   HIPCHK(hipMemcpyAsync(&h, bs, sizeof(h), hipMemcpyDeviceToHost, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
   st->site_flops = h.site_flops;
@@ -1798,6 +1810,65 @@ static int bicgstab_device_loop(qphix_solver *s, void *x, const SV &r0, const SV
   return QPHIX_OK;
 }
 
+// The CG iteration on M^+M (single flavour) with its scalars in device memory
+// (CgDState): M^+M p, |M p|^2, a (k_cgd_mid), r -= a M^+M p and x += a p with
+// |r|^2, the convergence test and b (k_cgd_post), p = b p + r; no host round trip.
+static MsState::HostView *g_cg_view = nullptr;  // pinned host, 2 entries
+static int cg_device_loop(qphix_solver *s, void *x, const SV &r, const SV &p, SV &mp, SV &mmp, int isign, int cb,
+                          double cp, double rsd_sq, unsigned long sf, qphix_solve_stats *st) {
+  qphix_op *M = s->M;
+  const qphix_geom *g = M->g;
+  if (!g_cg_view) HIPCHK(hipHostMalloc(&g_cg_view, 2 * sizeof(MsState::HostView), hipHostMallocCoherent));
+  CgDState h;
+  memset(&h, 0, sizeof(h));
+  h.c = cp;
+  h.cp = cp;
+  h.rsd_sq = rsd_sq;
+  h.site_flops = st->site_flops;
+  h.mv_apps = st->mv_apps;
+  h.sf = sf;
+  h.max_iters = s->max_iters;
+  h.progress = cp / rsd_sq;
+  h.hview = g_cg_view;
+  DevBuf dcs;
+  if (hipMalloc(&dcs.p, sizeof(CgDState)) != hipSuccess) return fail(QPHIX_ENOMEM, "CG state allocation");
+  CgDState *cs = (CgDState *)dcs.p;
+  HIPCHK(hipMemcpyAsync(cs, &h, sizeof(h), hipMemcpyHostToDevice, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  struct SkipScope {  // hop launches of this loop skip once the solve is done
+    explicit SkipScope(const int *q) { g_hop_skip = q; }
+    ~SkipScope() { g_hop_skip = nullptr; }
+  } skip_scope(&cs->done);
+  auto iteration = [&]() -> int {
+    TRY(op_apply_v(M, mp, cv(p), isign, cb));
+    TRY(op_apply_v(M, mmp, cv(mp), -isign, cb));
+    BlasArgs a = bargs();
+    a.x = mp[0];
+    TRY(run_blas_q(g, B_NORM2, a, nullptr, &cs->d_raw));
+    HIPCHK(cgd_mid_launch(cs, g_stream));
+    a = bargs();
+    a.x = mmp[0]; a.y = p[0]; a.o0 = r[0]; a.o1 = x;
+    TRY(run_blas_q(g, B_RMAMMP, a, cs->rmammp, &cs->r_raw));
+    HIPCHK(cgd_post_launch(cs, g_stream));
+    a = bargs();
+    a.x = r[0]; a.o0 = p[0];
+    TRY(run_blas_q(g, B_AYPX, a, cs->aypx, nullptr));
+    return QPHIX_OK;
+  };
+  TRY(device_iterations(s->dev_mode, s->max_iters + 1, iteration, g_cg_view));
+  HIPCHK(hipMemcpyAsync(&h, cs, sizeof(h), hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  st->site_flops = h.site_flops;
+  st->mv_apps = h.mv_apps;
+  st->rsd_sq_final = h.rsd_sq_final;
+  st->n_iters = h.n_iters;
+  st->status = h.status;
+  return QPHIX_OK;
+}
+
+// invbicgstab.h:86-262.  Counters as invbicgstab.h:158-240 counts them,
+// including the initial M x (skipped, not its count, when x is exactly zero);
+// the |r0|^2 of the iteration-0 test is not counted there either.
 static int solve_bicgstab(qphix_solver *s, void *const *x, const void *const *rhs, double rsd_target, int isign,
                           int verbose, int cb, int rtype, qphix_solve_stats *st) {
   qphix_op *M = s->M;
@@ -2646,26 +2717,13 @@ extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *c
 // The multishift iteration with its scalars in device memory (MsState): per
 // iteration the kernels a / as_s (k_ms_pre), the vector sweep, M^+M p0, |M p0|^2,
 // b (k_ms_mid), r += b M^+M p0 with |r|^2, then the zeta / beta recurrences and
-// the convergence test (k_ms_post); no host round trip. Mode 2 captures one
-// iteration in a HIP graph and replays it. The host enqueues iteration k + 1
-// before it reads iteration k's completion flag, so the GPU does not wait for
-// the host, except when the residual trend (MsState::progress of the last two
-// iterations read) says iteration k may be the last: then it waits for k
-// first, so that no full iteration is spent after the solve is done. Once done
-// the state is frozen; a closing sweep applies the x_s updates left pending.
-// Returns 0, -1 (not converged) or a status.
-struct MsGraph {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  ~MsGraph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    for (hipEvent_t e : ev)
-      if (e) (void)hipEventDestroy(e);
-  }
-};
+// the convergence test (k_ms_post); no host round trip (device_iterations runs
+// it, eagerly or replayed from a HIP graph). Once done the state is frozen; a
+// closing sweep applies the x_s updates left pending. Returns 0, -1 (not
+// converged) or a status.
 static MsState::HostView *g_ms_view = nullptr;  // pinned host, 2 entries
+static int device_iterations(int mode, int max_steps, const std::function<int()> &iteration,
+                             MsState::HostView *view);
 
 static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, const double *shifts, int isign, int cb,
                               const std::vector<void *> &r, const std::vector<void *> &p0,
@@ -2737,42 +2795,7 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
     return QPHIX_OK;
   };
   HIPCHK(ms_pre_launch(ms, g_stream));  // the first iteration's a, as_s
-  MsGraph gr;
-  HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
-  double prev = -1, prev2 = -1;  // progress of the last two iterations read
-  for (int k = 0;; k++) {
-    // the first iteration runs eagerly (it tunes any kernel not tuned yet),
-    // mode 2 captures the second one and replays it from then on
-    if (s->ms_mode == 2 && k == 1) {
-      HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
-      const int rc = iteration();
-      const hipError_t e = hipStreamEndCapture(g_stream, &gr.graph);
-      if (rc) return rc;
-      HIPCHK(e);
-      HIPCHK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
-    }
-    if (s->ms_mode == 2 && k >= 1) {
-      HIPCHK(hipGraphLaunch(gr.exec, g_stream));
-    } else {
-      TRY(iteration());
-    }
-    HIPCHK(hipEventRecord(gr.ev[k & 1], g_stream));
-    if (k > 0) {
-      HIPCHK(hipEventSynchronize(gr.ev[(k - 1) & 1]));
-      if (g_ms_view[(k - 1) & 1].done) break;
-      prev2 = prev;
-      prev = g_ms_view[(k - 1) & 1].progress;
-    }
-    // iteration k may end the solve (geometric extrapolation of the residual,
-    // or the iteration limit): wait for it before enqueueing another one
-    const bool last_near = (prev > 0 && prev2 > 0 && prev * (prev / prev2) < 1.5) || k + 2 >= s->max_iters;
-    if (last_near) {
-      HIPCHK(hipEventSynchronize(gr.ev[k & 1]));
-      if (g_ms_view[k & 1].done) break;
-    }
-    if (k > s->max_iters + 1) break;  // the state is done after max_iters - 1 iterations
-  }
+  TRY(device_iterations(s->ms_mode, s->max_iters, iteration, g_ms_view));
   TRY(sweep());  // the pending x_s updates (a no-op if the last iteration applied them)
   HIPCHK(ms_post_launch(ms, g_stream));
   HIPCHK(hipMemcpyAsync(&h, ms, sizeof(h), hipMemcpyDeviceToHost, g_stream));
@@ -2791,10 +2814,10 @@ extern "C" int qphix_solver_set_mode(qphix_solver *s, int mode) {
   if (mode < 0 || mode > 2) return fail(QPHIX_EDOMAIN, "solver mode %d (0, 1 or 2)", mode);
   if (s->kind == S_MINVCG) {
     s->ms_mode = mode;
-  } else if (s->kind == S_BICGSTAB) {
+  } else if (s->kind == S_BICGSTAB || s->kind == S_CG) {
     s->dev_mode = mode;
   } else {
-    return fail(QPHIX_EINVAL, "device-resident control exists for BiCGStab and multishift CG solvers");
+    return fail(QPHIX_EINVAL, "device-resident control exists for CG, BiCGStab and multishift CG solvers");
   }
   return QPHIX_OK;
 }

@@ -710,6 +710,60 @@ hipError_t bi_post_launch(BiState *st_dev, hipStream_t st) {
   return hipGetLastError();
 }
 
+// CG with its scalars in device memory (invcg.h:293-343): the host loop's
+// expressions, contraction off
+__device__ inline void cgd_freeze(CgDState *c, int status, int n_iters, double rsd) {
+  c->status = status;
+  c->n_iters = n_iters;
+  c->rsd_sq_final = rsd;
+  c->done = 1;
+  c->rmammp[7] = c->aypx[7] = 1.0;
+}
+__global__ void k_cgd_mid(CgDState *c) {
+#pragma clang fp contract(off)
+  if (c->done) return;
+  c->mv_apps += 2;
+  double d = 0;
+  d += c->d_raw;
+  c->site_flops += 4 * c->sf;
+  c->rmammp[0] = c->c / d;
+}
+__global__ void k_cgd_post(CgDState *c) {
+#pragma clang fp contract(off)
+  if (!c->done) {
+    double cp = 0;
+    cp += c->r_raw;
+    c->cp = cp;
+    c->site_flops += 12 * c->sf;
+    c->progress = cp / c->rsd_sq;
+    const int k = c->iters;
+    if (cp < c->rsd_sq) {
+      cgd_freeze(c, 0, k, cp);
+    } else {
+      c->aypx[0] = cp / c->c;
+      c->site_flops += 4 * c->sf;
+      c->c = cp;
+      if (k >= c->max_iters)
+        cgd_freeze(c, 5 /* QPHIX_ENOCONV */, c->max_iters, cp);
+      else
+        c->iters = k + 1;
+    }
+  }
+  MsState::HostView *v = c->hview + (c->posts & 1);
+  v->done = c->done;
+  v->progress = c->progress;
+  __threadfence_system();
+  c->posts++;
+}
+hipError_t cgd_mid_launch(CgDState *st_dev, hipStream_t st) {
+  k_cgd_mid<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+hipError_t cgd_post_launch(CgDState *st_dev, hipStream_t st) {
+  k_cgd_post<<<1, 1, 0, st>>>(st_dev);
+  return hipGetLastError();
+}
+
 // blas_site<P, B_AXPYNORM2> with the scalar read from device memory
 template <int P>
 __global__ void __launch_bounds__(kBlasThreads) k_axpy_norm2_dev(const double *b, const void *x, void *y, size_t vs,

@@ -143,6 +143,22 @@ hipError_t bi_mid2_launch(BiState *st_dev, hipStream_t st);
 // after the x/r update: |r|^2, convergence, the iteration count
 hipError_t bi_post_launch(BiState *st_dev, hipStream_t st);
 
+// CG state in device memory (invcg.h:88-456): c = |r|^2 of the iteration,
+// the reductions |M p|^2 and |r|^2, the scalar blocks of the two vector
+// updates (BlasArgs::dsc; [7] = skip once done), the counters
+struct CgDState {
+  double c, d_raw, r_raw, rsd_sq, cp;
+  double rmammp[8], aypx[8];
+  unsigned long site_flops, mv_apps, sf;
+  int iters, max_iters, done, status, n_iters, posts;
+  double rsd_sq_final, progress;
+  MsState::HostView *hview;
+};
+// after M^+M p and |M p|^2: a = c / d
+hipError_t cgd_mid_launch(CgDState *st_dev, hipStream_t st);
+// after r -= a M^+M p, x += a p, |r|^2: convergence, b = |r|^2 / c
+hipError_t cgd_post_launch(CgDState *st_dev, hipStream_t st);
+
 // y += (*b) x and |y|^2, the scalar read from device memory; partials as
 // blas_launch, the reduction finalized into *out
 hipError_t axpy_norm2_dev_launch(const GeomDesc &g, const double *b, const void *x, void *y, double *partials,

@@ -1,9 +1,9 @@
-"""BiCGStab with its complex scalars in device memory (mode 1, eager
-launches; mode 2, each iteration replayed from a HIP graph) against the loop
-with host scalars (mode 0, the reference's control flow,
-invbicgstab.h:150-256): solutions, iteration counts, final residuals,
-counters and statuses bit-identical for FP64, FP32, FP16 and BF16 storage,
-converging, stopped at MaxIters and breaking down; the FP64 solution checked
+"""BiCGStab and CG with their scalars in device memory (mode 1, eager
+launches; mode 2, each iteration replayed from a HIP graph) against the loops
+with host scalars (mode 0, the reference's control flow, invbicgstab.h:150-256,
+invcg.h:293-343): solutions, iteration counts, final residuals, counters and
+statuses bit-identical for FP64, FP32, FP16 and BF16 storage, converging,
+stopped at MaxIters and (BiCGStab) breaking down; the FP64 solutions checked
 against the oracle's operator; the mixed-precision defect correction with the
 device-scalar inner solver equal to the one with the host-scalar inner solver."""
 import numpy as np
@@ -116,3 +116,33 @@ def test_graph_mode_tunes_before_capture():
         out.append(((st.status, st.n_iters, st.rsd_sq_final, st.site_flops, st.mv_apps), x))
     assert out[0][0] == out[1][0] and out[0][0][0] == q.OK
     assert np.array_equal(out[0][1].download(1), out[1][1].download(1))
+
+
+def cg_solve(M, g, b, mode, max_iters, tol):
+    x = g.spinor()
+    st = q.InvCG(M, max_iters).set_mode(mode)(x, b, tol)
+    return (st.status, st.n_iters, st.rsd_sq_final, st.site_flops, st.mv_apps), x.download(1)
+
+
+@pytest.mark.parametrize("prec", [q.F64, q.F32, q.F16, q.BF16], ids=["f64", "f32", "f16", "bf16"])
+def test_cg_device_scalars_bit_identical(fields, prec):
+    g, M, _ = fields[prec]
+    bh = oracle.gen_spinor(L8, 180, 1)
+    b = g.spinor().upload(bh, 1)
+    tol = {q.F64: 1e-11, q.F32: 1e-6, q.F16: 1e-2, q.BF16: 1e-2}[prec]
+    res = [cg_solve(M, g, b, mode, 1000, tol) for mode in (0, 1, 2, 2)]
+    assert res[0][0][0] == q.OK and res[0][0][1] > 1
+    for stats, x in res[1:]:
+        assert stats == res[0][0]
+        assert np.array_equal(x, res[0][1])
+    if prec == q.F64:
+        B = b.download(1)
+        Mo = fields["orc"]
+        r = B - Mo(Mo(res[0][1], 1, 1), -1, 1)
+        assert np.linalg.norm(r) / np.linalg.norm(B) < 1.5e-11
+    # stopped at MaxIters
+    res = [cg_solve(M, g, b, mode, 3, 1e-14) for mode in (0, 1, 2)]
+    assert res[0][0][0] == q.ENOCONV
+    for stats, x in res[1:]:
+        assert stats == res[0][0]
+        assert np.array_equal(x, res[0][1])
