@@ -1695,7 +1695,7 @@ struct GraphRun {
       if (e) (void)hipEventDestroy(e);
   }
 };
-static MsState::HostView *g_bi_view = nullptr;  // pinned host, 2 entries
+static SolverView *g_bi_view = nullptr;  // pinned host, 2 entries
 
 // Runs `iteration` (kernels only, one solver iteration on g_stream) until the
 // device state it publishes to view[posts & 1] says done. The host enqueues
@@ -1705,7 +1705,7 @@ static MsState::HostView *g_bi_view = nullptr;  // pinned host, 2 entries
 // runs eagerly (it tunes any kernel not tuned yet); mode 2 captures the second
 // one in a HIP graph and replays it from then on.
 static int device_iterations(int mode, int max_steps, const std::function<int()> &iteration,
-                             MsState::HostView *view) {
+                             SolverView *view) {
   GraphRun gr;
   HIPCHK(hipEventCreateWithFlags(&gr.ev[0], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&gr.ev[1], hipEventDisableTiming));
@@ -1752,7 +1752,7 @@ static int bicgstab_device_loop(qphix_solver *s, void *x, const SV &r0, const SV
                                 unsigned long sf, qphix_solve_stats *st) {
   qphix_op *M = s->M;
   const qphix_geom *g = M->g;
-  if (!g_bi_view) HIPCHK(hipHostMalloc(&g_bi_view, 2 * sizeof(MsState::HostView), hipHostMallocCoherent));
+  if (!g_bi_view) HIPCHK(hipHostMalloc(&g_bi_view, 2 * sizeof(SolverView), hipHostMallocCoherent));
   BiState h;
   memset(&h, 0, sizeof(h));
   h.rho_prev[0] = h.alpha[0] = h.omega[0] = 1.0;
@@ -1813,12 +1813,12 @@ static int bicgstab_device_loop(qphix_solver *s, void *x, const SV &r0, const SV
 // The CG iteration on M^+M (single flavour) with its scalars in device memory
 // (CgDState): M^+M p, |M p|^2, a (k_cgd_mid), r -= a M^+M p and x += a p with
 // |r|^2, the convergence test and b (k_cgd_post), p = b p + r; no host round trip.
-static MsState::HostView *g_cg_view = nullptr;  // pinned host, 2 entries
+static SolverView *g_cg_view = nullptr;  // pinned host, 2 entries
 static int cg_device_loop(qphix_solver *s, void *x, const SV &r, const SV &p, SV &mp, SV &mmp, int isign, int cb,
                           double cp, double rsd_sq, unsigned long sf, qphix_solve_stats *st) {
   qphix_op *M = s->M;
   const qphix_geom *g = M->g;
-  if (!g_cg_view) HIPCHK(hipHostMalloc(&g_cg_view, 2 * sizeof(MsState::HostView), hipHostMallocCoherent));
+  if (!g_cg_view) HIPCHK(hipHostMalloc(&g_cg_view, 2 * sizeof(SolverView), hipHostMallocCoherent));
   CgDState h;
   memset(&h, 0, sizeof(h));
   h.c = cp;
@@ -2721,9 +2721,9 @@ extern "C" int qphix_solve_multi(qphix_solver *s, void *const x[], const void *c
 // it, eagerly or replayed from a HIP graph). Once done the state is frozen; a
 // closing sweep applies the x_s updates left pending. Returns 0, -1 (not
 // converged) or a status.
-static MsState::HostView *g_ms_view = nullptr;  // pinned host, 2 entries
+static SolverView *g_ms_view = nullptr;  // pinned host, 2 entries
 static int device_iterations(int mode, int max_steps, const std::function<int()> &iteration,
-                             MsState::HostView *view);
+                             SolverView *view);
 
 static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, const double *shifts, int isign, int cb,
                               const std::vector<void *> &r, const std::vector<void *> &p0,
@@ -2736,7 +2736,7 @@ static int minvcg_device_loop(qphix_solver *s, void *const x[], int n_shift, con
   qphix_op *M = s->M;
   const qphix_geom *g = M->g;
   const int nf = M->nflav;
-  if (!g_ms_view) HIPCHK(hipHostMalloc(&g_ms_view, 2 * sizeof(MsState::HostView), hipHostMallocCoherent));
+  if (!g_ms_view) HIPCHK(hipHostMalloc(&g_ms_view, 2 * sizeof(SolverView), hipHostMallocCoherent));
   MsState h;
   memset(&h, 0, sizeof(h));
   h.b = b;

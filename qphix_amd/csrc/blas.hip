@@ -496,7 +496,7 @@ __global__ void k_ms_mid(MsState *ms) {
 }
 
 __device__ inline void ms_publish(MsState *ms) {
-  MsState::HostView *v = ms->hview + (ms->posts & 1);
+  SolverView *v = ms->hview + (ms->posts & 1);
   v->done = ms->done;
   v->progress = ms->progress;
   __threadfence_system();
@@ -598,7 +598,7 @@ __device__ inline void bi_freeze(BiState *b, int status, int n_iters) {
   b->pupd[7] = b->supd[7] = b->rxupd[7] = 1.0;
 }
 __device__ inline void bi_publish(BiState *b) {
-  MsState::HostView *v = b->hview + (b->posts & 1);
+  SolverView *v = b->hview + (b->posts & 1);
   v->done = b->done;
   v->progress = b->progress;
   __threadfence_system();
@@ -749,7 +749,7 @@ __global__ void k_cgd_post(CgDState *c) {
         c->iters = k + 1;
     }
   }
-  MsState::HostView *v = c->hview + (c->posts & 1);
+  SolverView *v = c->hview + (c->posts & 1);
   v->done = c->done;
   v->progress = c->progress;
   __threadfence_system();

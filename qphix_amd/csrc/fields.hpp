@@ -83,6 +83,15 @@ struct MShiftArgs {
 };
 hipError_t mshift_launch(const GeomDesc &g, const MShiftArgs &m, hipStream_t st);
 
+// What a device-resident solver loop publishes at the end of every iteration
+// into view[posts & 1] of a pinned host array: the host reads the state of a
+// finished iteration without a copy command (progress: the residual over its
+// target, below 1 once converged)
+struct SolverView {
+  int done, pad;
+  double progress;
+};
+
 // Multishift CG state in device memory (minvcg.h:279-384): the FP64 scalars
 // and per-shift recurrences of one solve, updated by single-thread kernels
 // between the vector sweeps, so that an iteration needs no host round trip
@@ -100,12 +109,7 @@ struct MsState {
   // iteration (below 1 for a shift that converged there); lets the host
   // foresee the last iteration
   double progress;
-  // written by every k_ms_post into hview[posts & 1] (pinned host memory):
-  // the host reads the state of a finished iteration without a copy command
-  struct HostView {
-    int done, pad;
-    double progress;
-  } *hview;
+  SolverView *hview;
   int posts;
 };
 // the vector sweep of mshift_launch with a, zeta, as, beta, the masks and
@@ -132,7 +136,7 @@ struct BiState {
   unsigned long site_flops, mv_apps, sf;  // sf: the reference's unit, 12 x flavours
   int iters, max_iters, done, status, n_iters, posts;
   double progress;
-  MsState::HostView *hview;
+  SolverView *hview;
 };
 // start of iteration k: rho = <r0, r>, the breakdown test, beta, p-update scalars
 hipError_t bi_pre_launch(BiState *st_dev, hipStream_t st);
@@ -152,7 +156,7 @@ struct CgDState {
   unsigned long site_flops, mv_apps, sf;
   int iters, max_iters, done, status, n_iters, posts;
   double rsd_sq_final, progress;
-  MsState::HostView *hview;
+  SolverView *hview;
 };
 // after M^+M p and |M p|^2: a = c / d
 hipError_t cgd_mid_launch(CgDState *st_dev, hipStream_t st);
