@@ -232,7 +232,10 @@ extern "C" int qphix_geom_create(const int latt[4], int prec, int c12, qphix_geo
   g->d.prec = prec;
   g->d.c12 = c12 ? 1 : 0;
   g->d.vh = vh;
-  g->d.vs = (size_t)((g->d.vh + 255) / 256) * 256;
+  {
+    const long r = kSiteBlock > 256 ? kSiteBlock : 256;  // whole layout blocks and work-groups
+    g->d.vs = (size_t)((g->d.vh + r - 1) / r) * r;
+  }
   const int nr[4] = {kSpinorReals, gauge_reals(c12 != 0), kCloverReals, kFullCloverReals};
   for (int k = 0; k < 4; k++) {
     switch (prec) {

@@ -1,11 +1,11 @@
 // Block (multi-RHS) BLAS for block Krylov solvers: the Gram matrix of two sets
 // of spinor fields and the combination of one set by a small dense matrix.
 //
-// Both kernels walk the fields one 64-site block and one 16-byte word at a
-// time: the word of every right-hand side is staged in LDS with coalesced 1 KiB
-// loads, so each work-item then combines all right-hand sides of its site
-// without re-reading HBM. Gram products are summed in the storage precision
-// over one word of a 64-site block and then in FP64; per-work-group partials are
+// Both kernels walk the fields 64 sites and one 16-byte word at a time: the
+// word of every right-hand side is staged in LDS with coalesced 1 KiB loads, so
+// each work-item then combines all right-hand sides of its site without
+// re-reading HBM. Gram products are summed in the storage precision over one
+// word of 64 sites and then in FP64; per-work-group partials are
 // summed in a fixed order, so results do not depend on timing.
 #include "blockblas.hpp"
 
@@ -43,11 +43,11 @@ __global__ void __launch_bounds__(kGramThreads) k_block_gram(const BlockArgs ba,
         const int f = idx >> 6, l = idx & 63;
         T v[PW];
         if (f < na) {
-          Word<P>::template load<true>(ka->a[f], ((size_t)sb * W + w) * 64 + l, v);
+          Word<P>::template load<true>(ka->a[f], widx((long)sb * 64 + l, w, W), v);
 #pragma unroll
           for (int k = 0; k < PW; k++) la[(f * kGramPad + l) * PW + k] = v[k];
         } else {
-          Word<P>::template load<true>(ka->b[f - na], ((size_t)sb * W + w) * 64 + l, v);
+          Word<P>::template load<true>(ka->b[f - na], widx((long)sb * 64 + l, w, W), v);
 #pragma unroll
           for (int k = 0; k < PW; k++) lb[((f - na) * kGramPad + l) * PW + k] = v[k];
         }
@@ -135,12 +135,12 @@ __global__ void __launch_bounds__(kGramThreads) k_block_combine(const BlockArgs 
       for (int idx = tid; idx < na * 64; idx += kGramThreads) {
         const int f = idx >> 6, ll = idx & 63;
         T v[PW];
-        Word<P>::template load<false>(ka->a[f], ((size_t)sb * W + w) * 64 + ll, v);
+        Word<P>::template load<false>(ka->a[f], widx((long)sb * 64 + ll, w, W), v);
 #pragma unroll
         for (int k = 0; k < PW; k++) la[(f * 64 + ll) * PW + k] = v[k];
       }
       __syncthreads();
-      const size_t wi = ((size_t)sb * W + w) * 64 + l;
+      const size_t wi = widx((long)sb * 64 + l, w, W);
       for (int j = jb; j < nb; j += kGramThreads / 64) {
         T o[PW];
         const void *y = ka->y[j];

@@ -40,10 +40,14 @@ def test_geometry_validation_without_gpu():
     (q.BF16, 1, (52, 196, 148, 292)),
 ])
 def test_field_sizes(prec, c12, expect):
-    g = q.Geometry([8, 8, 8, 8], prec, c12)
-    # bytes per site of spinor / gauge (8 links) / clover / full clover
-    per_site = [b / 2048 for b in g.bytes]
+    # bytes per site of spinor / gauge (8 links) / clover / full clover, on a
+    # parity of 8192 sites (two whole 4096-site layout blocks)
+    g = q.Geometry([16, 16, 16, 4], prec, c12)
+    per_site = [b / 8192 for b in g.bytes]
     assert per_site == list(expect)
+    # a parity is padded to whole layout blocks: 8^4 (2048 sites) takes one
+    g = q.Geometry([8, 8, 8, 8], prec, c12)
+    assert [b / 4096 for b in g.bytes] == list(expect)
 
 
 def test_strerror():
