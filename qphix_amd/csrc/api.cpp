@@ -918,13 +918,15 @@ static void tune_rhs_tile(const GeomDesc &g, int kind, int isign, const HopArgs 
   // reservation the kernel does not use) keep the neighbours of the sites in
   // flight resident; the single-RHS traversal stays a candidate.
   const int ny = g.L[1], nz = g.L[2], nt = g.L[3];
-  static const int kTiles[][2] = {{4, 0}, {8, 0}, {2, 16}, {4, 8}, {6, 16}, {8, 16}};
+  // (bz, by) tiles; LDS reservations of 40 KiB (three work-groups per CU) and,
+  // for 256-thread work-groups, 64 KiB (two)
+  static const int kTiles[][2] = {{4, 0}, {8, 0}, {2, 16}, {4, 8}, {6, 8}, {6, 16}, {8, 16}};
   for (size_t t = 0; t < tiles.size() && t < 1; t++) {
     for (int blk : {128, 256})
       for (const auto &zy : kTiles)
-        for (int lds : {0, 40960}) {
+        for (int lds : {0, 40960, 65536}) {
           const int bz = zy[0], by = zy[1];
-          if (nz % bz || (by > 0 && ny % by) || nt < 2) continue;
+          if (nz % bz || (by > 0 && ny % by) || nt < 2 || (lds > 40960 && blk != 256)) continue;
           a.bz = bz;
           a.by = by;
           a.slabs = 1;
