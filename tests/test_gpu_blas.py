@@ -11,23 +11,27 @@ import qphix_amd as q
 
 pytestmark = pytest.mark.gpu
 
-L = [4, 4, 8, 8]
+# one 4096-site layout block, and a parity of 8,400 sites over three blocks
+# (the last one partial)
+LATS = [[4, 4, 8, 8], [10, 12, 14, 10]]
+PRECS = [q.F64, q.F32, q.F16, q.BF16]
 
 
 def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
 
 
-@pytest.fixture(scope="module", params=[q.F64, q.F32, q.F16, q.BF16], ids=["f64", "f32", "f16", "bf16"])
+@pytest.fixture(scope="module", params=[(p, L) for L in LATS for p in PRECS],
+                ids=[f"{n}-{'x'.join(map(str, L))}" for L in LATS for n in ("f64", "f32", "f16", "bf16")])
 def env(request):
-    prec = request.param
+    prec, L = request.param
     g = q.Geometry(L, prec, prec in (q.F16, q.BF16))
     tol = {q.F64: 1e-13, q.F32: 3e-7, q.F16: 2e-3, q.BF16: 1.6e-2}[prec]
     return g, tol
 
 
 def vec(g, seed):
-    d = g.spinor().upload(oracle.gen_spinor(L, seed, 0))
+    d = g.spinor().upload(oracle.gen_spinor(g.latt, seed, 0))
     return d, d.download()
 
 

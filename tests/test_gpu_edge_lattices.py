@@ -2,7 +2,9 @@
 extent 2 (the forward and backward neighbours are the same site), an odd
 number of checkerboarded x sites (Nxh = 3, 5, 7), parities smaller than one
 wavefront and parities that end in a partial wavefront (the cross-lane x / y
-exchange must fall back to loads there). Wilson, clover, clover
+exchange must fall back to loads there), and a parity of 8,400 sites that spans
+two whole 4096-site layout blocks and ends in a partial third (neighbours in
+other blocks, odd Nxh, rows straddling block ends). Wilson, clover, clover
 A chi - beta D psi and twisted-mass kernels in FP64, FP32 and FP16, both
 checkerboards and both signs, and an FP64 solve on the smallest lattice."""
 import numpy as np
@@ -14,7 +16,8 @@ import qphix_amd as q
 pytestmark = pytest.mark.gpu
 
 TOL = {q.F64: 1e-13, q.F32: 2e-6, q.F16: 5e-3, q.BF16: 4e-2}
-SHAPES = [[2, 2, 2, 2], [6, 2, 2, 4], [2, 4, 2, 6], [10, 6, 2, 2], [14, 6, 4, 2], [6, 10, 6, 4]]
+SHAPES = [[2, 2, 2, 2], [6, 2, 2, 4], [2, 4, 2, 6], [10, 6, 2, 2], [14, 6, 4, 2], [6, 10, 6, 4],
+          [10, 12, 14, 10]]
 PRECS = [(q.F64, False), (q.F32, True), (q.F16, True), (q.BF16, True)]
 
 
