@@ -755,7 +755,7 @@ __device__ inline void hop_body(const HopArgs &a, long k, const void *psi_p, voi
 // minimum waves per SIMD the register allocation must allow for the hop
 // kernels. None is imposed: with the 4096-site field blocks the compiler's own
 // allocation gives every FP16 / BF16 form four waves (at most 126 VGPRs) and
-// every FP32 form three, while an explicit floor of four made its scheduler
+// the FP32 forms three or four, while an explicit floor of four made its scheduler
 // spill the FP16 A chi - b D psi forms (9 VGPRs at isign +1). The issue-bound
 // FP16 kernels need those four waves to hide their load latency; a change
 // that takes them past 128 VGPRs costs a wave.
