@@ -348,6 +348,41 @@ def multi_rhs(q, D, g32, u32, inv32, nr, reps, per_ms, timer):
                          "mfma_utilisation_of_fp32_peak": round(issued / (ms_m * 1e-3) / 157.3e12, 4)}
     log(f"clover dslash f32 c12, {nm} rhs on matrix cores: {ms_m:.3f} ms ({ms_m / nm:.4f} ms per rhs), "
         f"MFMA {issued / (ms_m * 1e-3) / 1e12:.2f} TFLOP/s issued")
+    del psis, outs
+    # FP16 storage, 16 right-hand sides: the tuned vector tiles against the
+    # matrix-core form on v_mfma_f32_16x16x16f16 (one MFMA per site pair and
+    # hop: 8192 flops issued for 2 sites x 8 sources, 2304 of them useful)
+    g16 = q.Geometry(g32.latt, q.F16, True)
+    u16 = q.convert_gauge(g16, g32, u32)
+    inv16 = q.convert_clover(g16, g32, inv32)
+    D16 = q.Dslash(g16, t_boundary=-1.0)
+    psis = [q.gen_spinor(g16, 200 + r, 1) for r in range(nm)]
+    outs = [g16.spinor() for _ in range(nm)]
+
+    def time16():
+        D16.clov_dslash_multi(outs, psis, u16[0], inv16, 1, 0)
+        q.sync()
+        timer.start()
+        for _ in range(reps):
+            D16.clov_dslash_multi(outs, psis, u16[0], inv16, 1, 0)
+        return timer.stop() / reps
+
+    ms_v16 = time16()
+    q.call("qphix_tune_rhs", 8, nm)
+    ms_m16 = time16()
+    q.call("qphix_tune_rhs", *(RHS_TILE or (0, 0)))
+    issued16 = 4096.0 * vh * nm
+    b16 = (196 + 148 + nm * 104) * vh
+    out["f16_16rhs"] = {
+        "vector_tiles": {"ms": round(ms_v16, 4), "ms_per_rhs": round(ms_v16 / nm, 5),
+                         "gbs_algorithmic": round(b16 / (ms_v16 * 1e-3) / 1e9, 1)},
+        "mfma": {"ms": round(ms_m16, 4), "ms_per_rhs": round(ms_m16 / nm, 5),
+                 "gbs_algorithmic": round(b16 / (ms_m16 * 1e-3) / 1e9, 1),
+                 "mfma_tflops_issued": round(issued16 / (ms_m16 * 1e-3) / 1e12, 2),
+                 "mfma_tflops_useful": round(useful / (ms_m16 * 1e-3) / 1e12, 2),
+                 "mfma_utilisation_of_f16_peak": round(issued16 / (ms_m16 * 1e-3) / 2.5e15, 5)}}
+    log(f"clover dslash f16 c12, {nm} rhs: vector tiles {ms_v16:.3f} ms, matrix cores {ms_m16:.3f} ms "
+        f"(MFMA {issued16 / (ms_m16 * 1e-3) / 1e12:.2f} TFLOP/s issued)")
     return out
 
 

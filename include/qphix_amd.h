@@ -79,8 +79,9 @@ int qphix_tune_config(int mode, int block, int bz, int slabs);
 int qphix_tune_traversal(int block, int bz, int by, int slabs, int lds);
 /* multi-RHS tiling: reg sources per work-item (1, 2 or 4; each link and
  * clover block is loaded once for all of them; 8 selects the matrix-core
- * kernel for FP32 fields, 8 sources per work-group, other precisions then
- * fail with QPHIX_EHIP) and rt sources per launch (1..32, a multiple of reg);
+ * kernel for FP32 and FP16 (two-row) fields, 8 sources per work-group, other
+ * precisions then fail with QPHIX_EHIP) and rt sources per launch (1..32, a
+ * multiple of reg);
  * reg = 0 returns to autotuning. Clears the tuned multi-RHS configurations. */
 int qphix_tune_rhs(int reg, int rt);
 const char *qphix_tune_summary(void);

@@ -727,11 +727,13 @@ static hipError_t launch_hop_raw(const GeomDesc &g, int kind, int isign, const H
 }
 
 // launch of a.nrhs right-hand sides with `reg` of them per work-item; reg = 8
-// is the matrix-core kernel (FP32 storage), 8 right-hand sides per work-group
+// is the matrix-core kernel (FP32 or FP16 storage), 8 right-hand sides per
+// work-group
 static hipError_t launch_hop_tile_raw(const GeomDesc &g, int kind, int isign, const HopArgs &a, int block, int reg) {
   if (reg == 8) {
-    if (g.prec != P32) return hipErrorInvalidValue;
-    return launch_hop_mfma_f32(kind, g.c12 != 0, isign, a, g_stream);
+    if (g.prec == P32) return launch_hop_mfma_f32(kind, g.c12 != 0, isign, a, g_stream);
+    if (g.prec == P16) return launch_hop_mfma_f16(kind, g.c12 != 0, isign, a, g_stream);
+    return hipErrorInvalidValue;
   }
   switch (g.prec) {
     case P64: return launch_hop_tile_prec<P64>(kind, g.c12 != 0, isign, reg, a, block, g_stream);

@@ -898,5 +898,7 @@ hipError_t launch_hop_batch_prec(int kind, bool c12, int isign, const HopSeg *se
                                  const int *active, int grid, hipStream_t st);
 // matrix-core multi-RHS launch (FP32 storage, a.nrhs a multiple of 8)
 hipError_t launch_hop_mfma_f32(int kind, bool c12, int isign, const HopArgs &a, hipStream_t st);
+// matrix-core multi-RHS launch (FP16 storage, two-row links, a.nrhs a multiple of 8)
+hipError_t launch_hop_mfma_f16(int kind, bool c12, int isign, const HopArgs &a, hipStream_t st);
 
 }  // namespace qphix_amd

@@ -111,13 +111,16 @@ def test_register_tiled_multi_rhs(forced_rhs_tiling, reg, rt, prec, c12):
     for the matrix-core kernel, 11, so that a remainder goes through the
     one-per-work-item kernel) gives every right-hand side its single-RHS
     result: exactly for the vector kernels (same expressions, same rounding),
-    to FP32 rounding for the matrix-core kernel (FP32 fields only; its dot
-    products are summed in the matrix unit's order), for the Wilson, clover
-    and twisted-mass-clover members of the kernel family."""
+    to FP32 rounding for the FP32 matrix-core kernel (its dot products are
+    summed in the matrix unit's order) and to 3e-3 for the FP16 one (its
+    operands, the projected half spinors and the links with their rebuilt
+    third row, are rounded to half), for the Wilson, clover and
+    twisted-mass-clover members of the kernel family. FP64 and BF16 fields
+    have no matrix-core form."""
     L = [4, 4, 4, 8]
     mfma = reg == 8
     n = 11 if mfma else 5
-    if mfma and prec != q.F32:
+    if mfma and prec not in (q.F32, q.F16):
         q.call("qphix_tune_rhs", reg, rt)
         g = q.Geometry(L, prec, c12)
         D = q.Dslash(g, t_boundary=-1.0)
@@ -147,7 +150,8 @@ def test_register_tiled_multi_rhs(forced_rhs_tiling, reg, rt, prec, c12):
             single(one, r)
             got, exp = outs[r].download(cb), one.download(cb)
             if mfma:
-                assert np.linalg.norm(got - exp) <= 2e-6 * np.linalg.norm(exp), (r, reg, rt)
+                tol = 2e-6 if prec == q.F32 else 3e-3
+                assert np.linalg.norm(got - exp) <= tol * np.linalg.norm(exp), (r, reg, rt)
             else:
                 assert np.array_equal(got, exp), (r, reg, rt)
 
